@@ -1,0 +1,50 @@
+"""Convert the reference's own golden files for the TPFA path into small .npz
+fixtures under this folder.
+
+Only data is read (MATLAB .mat via scipy.io.loadmat, NumPy .npz with
+allow_pickle=False); nothing from the reference is imported or executed.
+Sources (relative to the reference checkout):
+
+* examples/validation/ref_p2_{qfs,bc,wells,gravity}.mat  - MRST incompTPFA and
+  computeTrans outputs (cases rebuilt in examples/validation/validate_phase2.py)
+* examples/validation/ref_p5_ow.mat  - MRST oil-water waterflood
+  (case in examples/validation/validate_phase5.py, dump_phase5_reference.m)
+* paired_waterflood_pymrst_results.npz - PyMRST's 15x15x3 waterflood
+  (case in pymrst/modules/ad_core/examples/paired_heterogeneous_waterflood.py)
+* examples/validation/ref_p4_ad.mat - MRST compressible single-phase AD run
+* examples/validation/ref_ad_blackoil_phase_flux.mat - per-face black-oil flux
+
+Run:  python tests/golden/make_fixtures.py [/path/to/reference]
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+import scipy.io
+
+HERE = Path(__file__).parent
+
+
+def _mat(path):
+    d = scipy.io.loadmat(path, squeeze_me=True)
+    return {k: np.asarray(v, dtype=float) for k, v in d.items() if not k.startswith("__")}
+
+
+def main(ref_root="/root/reference"):
+    ref = Path(ref_root)
+    val = ref / "examples" / "validation"
+    for case in ("qfs", "bc", "wells", "gravity"):
+        np.savez_compressed(HERE / f"p2_{case}.npz", **_mat(val / f"ref_p2_{case}.mat"))
+    np.savez_compressed(HERE / "p5_ow.npz", **_mat(val / "ref_p5_ow.mat"))
+    np.savez_compressed(HERE / "p4_ad.npz", **_mat(val / "ref_p4_ad.mat"))
+    np.savez_compressed(HERE / "bo_phase_flux.npz",
+                        **_mat(val / "ref_ad_blackoil_phase_flux.mat"))
+    pw = np.load(ref / "paired_waterflood_pymrst_results.npz", allow_pickle=False)
+    np.savez_compressed(HERE / "paired_waterflood.npz",
+                        **{k: np.asarray(pw[k]) for k in pw.files})
+    for f in sorted(HERE.glob("*.npz")):
+        print(f.name, f.stat().st_size)
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])
