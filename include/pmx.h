@@ -1,0 +1,122 @@
+/*
+ * pmx -- C ABI of the MI355X-native TPFA Newton step.
+ *
+ * Every array argument documented "device" is a caller-owned device pointer
+ * (e.g. a torch tensor's data_ptr()); "host" arrays are read during the call
+ * only.  One handle is bound to one HIP stream and must not be used from two
+ * host threads at once; every call is ordered on that stream.  Functions
+ * return 0 on success, 1 when an iterative solve did not converge, and a
+ * negative value on error (message from pmx_last_error()).
+ *
+ * Vector layouts on the device:
+ *   cell unknowns   cell-major interleaved: (p_0, sW_0, p_1, sW_1, ...)
+ *   well unknowns   well-major after the cells: (qWs_0, qOs_0, bhp_0, ...)
+ *   residual rows   (water_0, oil_0, ...), then (eq_w, eq_o, ctrl) per well
+ * The reference orders the same unknowns variable-blocked,
+ * [p(nc), sW(nc), qWs(nw), qOs(nw), bhp(nw)]; the Python layer permutes.
+ */
+#ifndef PMX_H
+#define PMX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmx_ow pmx_ow;
+
+typedef struct pmx_solver_options {
+  double rtol;              /* ||b - J x|| <= rtol ||b|| */
+  int32_t restart;
+  int32_t maxiter;
+  int32_t precision;        /* 64: FP64 preconditioner, 32: FP32 preconditioner */
+  int32_t use_pressure_stage;
+  int32_t pre_smooth, post_smooth;
+  double jacobi_omega;
+  double coarse_alpha;
+  double theta;             /* aggregation strength threshold */
+  int32_t coarse_size;
+  int32_t pairwise_passes;
+  int32_t reorthogonalize;
+  int32_t verbose;
+} pmx_solver_options;
+
+typedef struct pmx_solve_info {
+  int32_t iterations, restarts;
+  double rel_residual, setup_ms, solve_ms;
+  int32_t levels, coarse_n;
+} pmx_solve_info;
+
+/* Last error message of the calling thread. */
+const char* pmx_last_error(void);
+/* Default solver options (rtol 1e-8, restart 50, maxiter 200: the reference's
+ * CPRSolver defaults, pymrst/modules/ad_core/solvers/linear_solvers.py:113-120). */
+void pmx_solver_defaults(pmx_solver_options* o);
+
+/* ---- oil-water model (TwoPhaseOilWaterModelAD,
+ *      pymrst/modules/ad_blackoil/models/two_phase_oil_water_model.py:79-657) */
+
+/* fluid[10] = {mu_w, mu_o, rhoS_w, rhoS_o, n_w, n_o, c_w, c_o, p_ref, c_r};
+ * c_r < 0 means no rock compressibility (the legacy model's constant pv). */
+int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out);
+int pmx_ow_destroy(pmx_ow* h);
+
+/* Topology: nc cells, nconn internal connections N[2*nconn] (host int32,
+ * setup_operators_tpfa.py:98-110), T[nconn] transmissibilities, gdz[nconn] =
+ * g (z[N1] - z[N0]) (two_phase_oil_water_model.py:157-162), pv[nc] pore
+ * volumes.  All host arrays. */
+int pmx_ow_set_topology(pmx_ow* h, int32_t nc, int32_t nconn, const int32_t* N, const double* T,
+                        const double* gdz, const double* pv);
+
+/* Wells (add_well, pymrst/params/wells_and_bc/add_well.py:16-126): perf_ptr[nw+1],
+ * perf_cells[nperf], WI[nperf], type[nw] (0 bhp, 1 rate), val[nw],
+ * compi[2*nw].  Host arrays. */
+int pmx_ow_set_wells(pmx_ow* h, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells,
+                     const double* WI, const int32_t* type, const double* val, const double* compi);
+
+/* Residual and Jacobian at (p, sW, qWs, qOs, bhp) with the previous state
+ * (p0, sW0), time step dt and per-connection cdp[nperf] (device; may be NULL
+ * for zeros).  All state arrays device.  Replaces assemble + combine_equations
+ * (two_phase_oil_water_model.py:493-546, utils/adi/operators.py:204-211). */
+int pmx_ow_assemble(pmx_ow* h, const double* p, const double* sw, const double* qws, const double* qos,
+                    const double* bhp, const double* p0, const double* sw0, double dt, const double* cdp);
+
+/* Copy the last residual (2*nc + 3*nw, device layout) into out (device). */
+int pmx_ow_residual(pmx_ow* h, double* out);
+
+/* Convergence reductions of the last residual (_converged,
+ * two_phase_oil_water_model.py:549-567): out7 (host) = {max|r_w/pv|,
+ * max|r_o/pv|, sum r_w, sum r_o, sum 1/b_w(p), sum 1/b_o(p), sum p};
+ * wells_out (host, 3*nw) = well rows of the residual. */
+int pmx_ow_convergence(pmx_ow* h, const double* p, double* out7, double* wells_out);
+
+/* Solve J dx = -r (CPR-FGMRES) into dx (device, n = 2*nc + 3*nw); pmean is
+ * the mean pressure for the quasi-IMPES weights.  Replaces
+ * _solve_increment / CPRSolver.solve (linear_solvers.py:426-545). */
+int pmx_ow_solve(pmx_ow* h, const pmx_solver_options* o, double pmean, double* dx, pmx_solve_info* info);
+
+/* Apply x += sign*dx with the Newton update limits (step,
+ * two_phase_oil_water_model.py:629-651).  All device. */
+int pmx_ow_update(pmx_ow* h, const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* sw,
+                  double* qws, double* qos, double* bhp);
+
+/* y = J x for the last Jacobian (device vectors of length 2*nc + 3*nw). */
+int pmx_ow_matvec(pmx_ow* h, const double* x, double* y);
+
+/* Export the last Jacobian: block pattern (host rowptr[nc+1], col[nnzb]) and
+ * values (device, nnzb*4 row-major 2x2 blocks).  Either pointer may be NULL. */
+int pmx_ow_nnzb(pmx_ow* h, int32_t* nnzb);
+int pmx_ow_export_pattern(pmx_ow* h, int32_t* rowptr, int32_t* col);
+int pmx_ow_export_blocks(pmx_ow* h, double* vals);
+
+/* Connection surface rates (cq_w, cq_o) per perforation of the last
+ * assembly, into host out[2*nperf]. */
+int pmx_ow_connection_rates(pmx_ow* h, double* out);
+
+/* One preconditioner application z = M^{-1} r after a solve's setup (device). */
+int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+"""ctypes binding of the pmx C ABI (include/pmx.h).
+
+torch is imported first so that the library binds to the HIP runtime torch
+already loaded (both carry the soname libamdhip64.so.7).  A missing library
+is an error: there is no CPU fallback for the device path.
+"""
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime the library links against)
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "_pmx.so"
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [
+        ("rtol", ctypes.c_double),
+        ("restart", ctypes.c_int32),
+        ("maxiter", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("use_pressure_stage", ctypes.c_int32),
+        ("pre_smooth", ctypes.c_int32),
+        ("post_smooth", ctypes.c_int32),
+        ("jacobi_omega", ctypes.c_double),
+        ("coarse_alpha", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("coarse_size", ctypes.c_int32),
+        ("pairwise_passes", ctypes.c_int32),
+        ("reorthogonalize", ctypes.c_int32),
+        ("verbose", ctypes.c_int32),
+    ]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("restarts", ctypes.c_int32),
+        ("rel_residual", ctypes.c_double),
+        ("setup_ms", ctypes.c_double),
+        ("solve_ms", ctypes.c_double),
+        ("levels", ctypes.c_int32),
+        ("coarse_n", ctypes.c_int32),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+_D = ctypes.c_double
+
+#: name -> (restype, argtypes); every symbol declared in include/pmx.h
+SIGNATURES = {
+    "pmx_last_error": (ctypes.c_char_p, []),
+    "pmx_solver_defaults": (None, [ctypes.POINTER(SolverOptions)]),
+    "pmx_ow_create": (ctypes.c_int, [_I, _P, _P, ctypes.POINTER(_P)]),
+    "pmx_ow_destroy": (ctypes.c_int, [_P]),
+    "pmx_ow_set_topology": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P]),
+    "pmx_ow_set_wells": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "pmx_ow_assemble": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _P]),
+    "pmx_ow_residual": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_convergence": (ctypes.c_int, [_P, _P, _P, _P]),
+    "pmx_ow_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _D, _P, ctypes.POINTER(SolveInfo)]),
+    "pmx_ow_update": (ctypes.c_int, [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P]),
+    "pmx_ow_matvec": (ctypes.c_int, [_P, _P, _P]),
+    "pmx_ow_nnzb": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "pmx_ow_export_pattern": (ctypes.c_int, [_P, _P, _P]),
+    "pmx_ow_export_blocks": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_connection_rates": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
+}
+
+_lib = None
+
+
+class PmxError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise PmxError(f"pmx HIP library not built: {LIB_PATH} missing "
+                       "(run `python -m pymrst_amd.build`)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = "pmx call") -> int:
+    """Raise on a negative status; return 0/1 (converged / not converged)."""
+    if rc < 0:
+        msg = load().pmx_last_error().decode(errors="replace")
+        raise PmxError(f"{what} failed: {msg}")
+    return rc
+
+
+def ptr(t) -> ctypes.c_void_p:
+    """Device or host pointer of a tensor / numpy array (None -> NULL)."""
+    if t is None:
+        return ctypes.c_void_p(None)
+    if isinstance(t, torch.Tensor):
+        assert t.is_contiguous(), "tensor must be contiguous"
+        return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.ctypes.data)
+
+
+def solver_defaults() -> SolverOptions:
+    o = SolverOptions()
+    load().pmx_solver_defaults(ctypes.byref(o))
+    return o

@@ -1,0 +1,251 @@
+// C ABI entry points (include/pmx.h).  Exceptions never cross the boundary:
+// each entry point converts them into a negative status and a message.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/pmx.h"
+#include "ow.h"
+
+using namespace pmx;
+
+struct pmx_ow {
+  int device = 0;
+  std::unique_ptr<OWModelDev> m;
+  pmx_solver_options last_opts{};
+};
+
+static thread_local std::string g_err;
+
+#define PMX_TRY(body)                       \
+  try {                                     \
+    body;                                   \
+    return 0;                               \
+  } catch (const std::exception& e) {       \
+    g_err = e.what();                       \
+    return -1;                              \
+  } catch (...) {                           \
+    g_err = "unknown error";                \
+    return -1;                              \
+  }
+
+static LinearSolverOptions to_opts(const pmx_solver_options* o) {
+  LinearSolverOptions r;
+  if (!o) return r;
+  r.rtol = o->rtol;
+  r.restart = o->restart;
+  r.maxiter = o->maxiter;
+  r.precision = o->precision;
+  r.use_pressure_stage = o->use_pressure_stage;
+  r.pre_smooth = o->pre_smooth;
+  r.post_smooth = o->post_smooth;
+  r.jacobi_omega = o->jacobi_omega;
+  r.coarse_alpha = o->coarse_alpha;
+  r.theta = o->theta;
+  r.coarse_size = o->coarse_size;
+  r.pairwise_passes = o->pairwise_passes;
+  r.reorthogonalize = o->reorthogonalize;
+  r.verbose = o->verbose;
+  return r;
+}
+
+static void bind(pmx_ow* h) { PMX_HIP(hipSetDevice(h->device)); }
+
+extern "C" {
+
+const char* pmx_last_error(void) { return g_err.c_str(); }
+
+void pmx_solver_defaults(pmx_solver_options* o) {
+  LinearSolverOptions d;
+  o->rtol = d.rtol;
+  o->restart = d.restart;
+  o->maxiter = d.maxiter;
+  o->precision = d.precision;
+  o->use_pressure_stage = d.use_pressure_stage;
+  o->pre_smooth = d.pre_smooth;
+  o->post_smooth = d.post_smooth;
+  o->jacobi_omega = d.jacobi_omega;
+  o->coarse_alpha = d.coarse_alpha;
+  o->theta = d.theta;
+  o->coarse_size = d.coarse_size;
+  o->pairwise_passes = d.pairwise_passes;
+  o->reorthogonalize = d.reorthogonalize;
+  o->verbose = d.verbose;
+}
+
+int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {
+  PMX_TRY({
+    PMX_CHECK(out != nullptr && fluid != nullptr, "null argument");
+    auto h = std::make_unique<pmx_ow>();
+    h->device = device;
+    bind(h.get());
+    h->m = std::make_unique<OWModelDev>((hipStream_t)hip_stream);
+    OWFluid& f = h->m->fluid;
+    f.mu_w = fluid[0];
+    f.mu_o = fluid[1];
+    f.rho_w = fluid[2];
+    f.rho_o = fluid[3];
+    f.n_w = fluid[4];
+    f.n_o = fluid[5];
+    f.c_w = fluid[6];
+    f.c_o = fluid[7];
+    f.p_ref = fluid[8];
+    f.c_r = fluid[9] >= 0 ? fluid[9] : 0.0;
+    f.has_cr = fluid[9] >= 0 ? 1 : 0;
+    pmx_solver_defaults(&h->last_opts);
+    *out = h.release();
+  })
+}
+
+int pmx_ow_destroy(pmx_ow* h) {
+  PMX_TRY({
+    if (h) {
+      bind(h);
+      delete h;
+    }
+  })
+}
+
+int pmx_ow_set_topology(pmx_ow* h, int32_t nc, int32_t nconn, const int32_t* N, const double* T, const double* gdz,
+                        const double* pv) {
+  PMX_TRY({
+    PMX_CHECK(h && N && T && gdz && pv, "null argument");
+    bind(h);
+    h->m->set_topology(nc, nconn, N, T, gdz, pv);
+  })
+}
+
+int pmx_ow_set_wells(pmx_ow* h, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells, const double* WI,
+                     const int32_t* type, const double* val, const double* compi) {
+  PMX_TRY({
+    PMX_CHECK(h && (nw == 0 || (perf_ptr && perf_cells && WI && type && val && compi)), "null argument");
+    bind(h);
+    for (int w = 0; w < nw; ++w) PMX_CHECK(type[w] == 0 || type[w] == 1, "unsupported well control");
+    h->m->set_wells(nw, perf_ptr, perf_cells, WI, type, val, compi);
+  })
+}
+
+int pmx_ow_assemble(pmx_ow* h, const double* p, const double* sw, const double* qws, const double* qos,
+                    const double* bhp, const double* p0, const double* sw0, double dt, const double* cdp) {
+  PMX_TRY({
+    PMX_CHECK(h && p && sw && p0 && sw0, "null argument");
+    PMX_CHECK(dt > 0, "dt must be positive");
+    bind(h);
+    h->m->assemble(p, sw, qws, qos, bhp, p0, sw0, dt, cdp);
+  })
+}
+
+int pmx_ow_residual(pmx_ow* h, double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && out, "null argument");
+    bind(h);
+    PMX_HIP(hipMemcpyAsync(out, h->m->r.p, sizeof(double) * h->m->S.n(), hipMemcpyDeviceToDevice, h->m->stream));
+  })
+}
+
+int pmx_ow_convergence(pmx_ow* h, const double* p, double* out7, double* wells_out) {
+  PMX_TRY({
+    PMX_CHECK(h && p && out7, "null argument");
+    bind(h);
+    h->m->convergence(p, out7, wells_out);
+  })
+}
+
+int pmx_ow_solve(pmx_ow* h, const pmx_solver_options* o, double pmean, double* dx, pmx_solve_info* info) {
+  try {
+    PMX_CHECK(h && dx, "null argument");
+    bind(h);
+    SolveInfo si;
+    LinearSolverOptions lo = to_opts(o);
+    if (o) h->last_opts = *o;
+    const int st = h->m->solve(lo, dx, si, pmean);
+    if (info) {
+      info->iterations = si.iterations;
+      info->restarts = si.restarts;
+      info->rel_residual = si.rel_residual;
+      info->setup_ms = si.setup_ms;
+      info->solve_ms = si.solve_ms;
+      info->levels = si.levels;
+      info->coarse_n = si.coarse_n;
+    }
+    if (st < 0) {
+      g_err = "linear solve produced a non-finite residual";
+      return -2;
+    }
+    return st;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int pmx_ow_update(pmx_ow* h, const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* sw,
+                  double* qws, double* qos, double* bhp) {
+  PMX_TRY({
+    PMX_CHECK(h && dx && p && sw, "null argument");
+    bind(h);
+    h->m->update(dx, sign, ds_max, dp_max_rel, p, sw, qws, qos, bhp);
+  })
+}
+
+int pmx_ow_matvec(pmx_ow* h, const double* x, double* y) {
+  PMX_TRY({
+    PMX_CHECK(h && x && y, "null argument");
+    bind(h);
+    DArr<double> hs;
+    hs.alloc((size_t)std::max(h->m->nw, 1) * kMaxMix);
+    system_matvec(h->m->S, x, y, hs.p, h->m->stream);
+    PMX_HIP(hipStreamSynchronize(h->m->stream));
+  })
+}
+
+int pmx_ow_nnzb(pmx_ow* h, int32_t* nnzb) {
+  PMX_TRY({
+    PMX_CHECK(h && nnzb, "null argument");
+    *nnzb = h->m->S.nnzb;
+  })
+}
+
+int pmx_ow_export_pattern(pmx_ow* h, int32_t* rowptr, int32_t* col) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    bind(h);
+    auto& S = h->m->S;
+    if (rowptr)
+      PMX_HIP(hipMemcpyAsync(rowptr, S.rowptr.p, sizeof(int) * (S.nc + 1), hipMemcpyDeviceToHost, h->m->stream));
+    if (col) PMX_HIP(hipMemcpyAsync(col, S.col.p, sizeof(int) * S.nnzb, hipMemcpyDeviceToHost, h->m->stream));
+    PMX_HIP(hipStreamSynchronize(h->m->stream));
+  })
+}
+
+int pmx_ow_export_blocks(pmx_ow* h, double* vals) {
+  PMX_TRY({
+    PMX_CHECK(h && vals, "null argument");
+    bind(h);
+    auto& S = h->m->S;
+    PMX_HIP(hipMemcpyAsync(vals, S.val.p, sizeof(double) * 4 * S.nnzb, hipMemcpyDeviceToDevice, h->m->stream));
+  })
+}
+
+int pmx_ow_connection_rates(pmx_ow* h, double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && out, "null argument");
+    bind(h);
+    auto& m = *h->m;
+    if (m.S.nperf > 0) {
+      PMX_HIP(hipMemcpyAsync(out, m.rq.p, sizeof(double) * 2 * m.S.nperf, hipMemcpyDeviceToHost, m.stream));
+      PMX_HIP(hipStreamSynchronize(m.stream));
+    }
+  })
+}
+
+int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z) {
+  PMX_TRY({
+    PMX_CHECK(h && r && z, "null argument");
+    bind(h);
+    LinearSolverOptions lo = to_opts(o ? o : &h->last_opts);
+    cpr_apply(h->m->cpr, h->m->S, r, z, lo, h->m->stream);
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// Device linear systems of the TPFA Newton step: a cell-block BCSR matrix
+// coupled to a small well tail, the CPR two-stage preconditioner (pressure
+// aggregation AMG + cell block-Jacobi) and right-preconditioned flexible
+// GMRES.  All vectors are device arrays in the layout
+//   [cell 0: B unknowns][cell 1: B unknowns] ... [well 0: M unknowns] ...
+// i.e. cell-major interleaved cell unknowns followed by well-major well
+// unknowns.
+#pragma once
+#include "pmx_internal.h"
+
+namespace pmx {
+
+constexpr int kMaxB = 3;
+constexpr int kMaxM = 4;
+constexpr int kMaxMix = 3;
+
+// Jacobian of one Newton iteration.
+//   J = [ A   B ]   A: nc x nc blocks of BxB (BCSR)
+//       [ C   D ]   D: block-diagonal, one MxM block per well
+// Well coupling is stored per perforation j (cell c_j, well w_j):
+//   Bp[j] (BxM)  d cell-eq(c_j) / d wellvars(w_j)
+//   Cp[j] (MxB)  d wellrow(w_j) / d cellvars(c_j)
+//   Um[j] (BxX), V[j] (XxB): rank-X coupling between perforations of one well
+//     (mixture terms): A += sum_w sum_{j,i in w} Um[j] V[i] at block (c_j, c_i)
+struct BlockSystem {
+  int B = 2, M = 3, X = 2;
+  int nc = 0, nnzb = 0;
+  int nw = 0, nperf = 0;
+  bool has_mix = false;
+  DArr<int> rowptr, col, diag;        // BCSR pattern
+  DArr<double> val;                   // nnzb * B * B (row-major blocks)
+  // wells
+  DArr<int> perf_cell, perf_well, well_ptr;  // perforations grouped by well
+  DArr<int> pc_cell, pc_ptr, pc_perf;        // unique perforated cells -> perforations
+  int npc = 0;
+  DArr<double> Bp, Cp, Um, V, Dp;
+  size_t n() const { return (size_t)nc * B + (size_t)nw * M; }
+  void set_pattern(const HostPattern& pat, hipStream_t s);
+  void set_wells(int nw_, const std::vector<int>& perf_ptr, const std::vector<int>& perf_cells, hipStream_t s);
+};
+
+// y = J x  (double)
+void system_matvec(const BlockSystem& S, const double* x, double* y, double* scratch_h, hipStream_t s);
+
+struct LinearSolverOptions {
+  double rtol = 1e-8;
+  int restart = 50;
+  int maxiter = 200;
+  int precision = 64;        // 64: FP64 preconditioner; 32: FP32 preconditioner
+  int use_pressure_stage = 1;
+  int pre_smooth = 1, post_smooth = 1;
+  double jacobi_omega = 0.67;
+  double coarse_alpha = 1.0; // coarse-correction scaling
+  int amg_cycles = 1;
+  double theta = 0.25;
+  int coarse_size = 128;
+  int pairwise_passes = 0;
+  int reorthogonalize = 1;
+  int verbose = 0;
+};
+
+struct SolveInfo {
+  int iterations = 0;
+  int restarts = 0;
+  double rel_residual = 0.0;
+  double setup_ms = 0.0, solve_ms = 0.0;
+  int levels = 0;
+  int coarse_n = 0;
+};
+
+class CprSolver;   // opaque (linsys.hip)
+
+CprSolver* cpr_create();
+void cpr_destroy(CprSolver*);
+// Static aggregation hierarchy from pressure-graph weights per BCSR entry of S.
+void cpr_build_hierarchy(CprSolver*, const BlockSystem& S, const std::vector<double>& weights,
+                         const LinearSolverOptions& o, hipStream_t s);
+bool cpr_has_hierarchy(const CprSolver*);
+// Numeric setup for the current values of S (pressure weights: w_dev[B]).
+void cpr_setup(CprSolver*, const BlockSystem& S, const double* w_dev, const LinearSolverOptions& o,
+               hipStream_t s);
+// Solve J x = b (FGMRES); returns 0 converged, 1 not converged.
+int cpr_solve(CprSolver*, const BlockSystem& S, const double* b, double* x, const LinearSolverOptions& o,
+              SolveInfo& info, hipStream_t s);
+// z = M^{-1} r (one preconditioner application), exposed for tests.
+void cpr_apply(CprSolver*, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
+               hipStream_t s);
+
+// Helpers used by several sources.
+int grid_for(size_t threads, int block = 256);
+double device_dot(const double* a, const double* b, size_t n, double* partial, hipStream_t s);
+
+}  // namespace pmx

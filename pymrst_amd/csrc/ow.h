@@ -1,0 +1,43 @@
+// Device oil-water model (see ow.hip).
+#pragma once
+#include "linsys.h"
+
+namespace pmx {
+
+// init_simple_adi_fluid for phases 'WO': b_a = exp(c_a (p - p_ref)) (1 when
+// c_a == 0), kr_w = sW^n_w, kr_o = (1-sW)^n_o, constant viscosities; with
+// has_cr the pore volume is multiplied by 1 + c_r (p - p_ref).
+struct OWFluid {
+  double mu_w, mu_o, rho_w, rho_o, n_w, n_o, c_w, c_o, p_ref, c_r;
+  int has_cr;
+};
+
+class OWModelDev {
+ public:
+  explicit OWModelDev(hipStream_t s);
+  ~OWModelDev();
+  void set_topology(int nc, int nconn, const int* N, const double* T, const double* gdz, const double* pv);
+  void set_wells(int nw, const int* perf_ptr, const int* perf_cells, const double* WI, const int* types,
+                 const double* vals, const double* compi);
+  void assemble(const double* p, const double* s, const double* qws, const double* qos, const double* bhp,
+                const double* p0, const double* s0, double dt, const double* cdp_dev);
+  void convergence(const double* p, double* out7, double* wells_out);
+  int solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean);
+  void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
+              double* qos, double* bhp);
+
+  hipStream_t stream;
+  OWFluid fluid{};
+  int nc = 0, nconn = 0, nw = 0;
+  BlockSystem S;
+  DArr<int> bconn;
+  DArr<double> bT, bG, pv;
+  DArr<double> WI, cdp, wval, compi, rq, dq;
+  DArr<int> wtype, mixflag;
+  std::vector<int> host_types;
+  DArr<double> r, rhs, w_cpr, conv_partial;
+  std::vector<double> hier_weights;
+  CprSolver* cpr = nullptr;
+};
+
+}  // namespace pmx

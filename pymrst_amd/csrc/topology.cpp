@@ -1,0 +1,271 @@
+// Host-side construction of the static block pattern of a TPFA operator and
+// of the aggregation hierarchy used by the pressure AMG.  Runs once per
+// topology; everything that changes per Newton iteration is on the device.
+//
+// The TPFA pattern is the one the reference's Jacobian has on its cell-cell
+// blocks: the diagonal plus (N0,N1) and (N1,N0) for every internal
+// connection (pymrst/modules/ad_core/utils/setup_operators_tpfa.py:158-209;
+// Grad = -C x, Div = C' x).
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "pmx_internal.h"
+
+namespace pmx {
+
+void tpfa_pattern(int n, int nconn, const int* N, HostPattern& pat, std::vector<int>& entry_conn) {
+  PMX_CHECK(n >= 0 && nconn >= 0, "negative size");
+  std::vector<int> deg(n, 1);
+  for (int f = 0; f < nconn; ++f) {
+    int a = N[2 * f], b = N[2 * f + 1];
+    PMX_CHECK(a >= 0 && a < n && b >= 0 && b < n && a != b, "connection list holds an invalid cell pair");
+    deg[a]++;
+    deg[b]++;
+  }
+  pat.n = n;
+  pat.rowptr.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) pat.rowptr[i + 1] = pat.rowptr[i] + deg[i];
+  int nnz = pat.rowptr[n];
+  std::vector<std::pair<int, int>> ent(nnz);  // (col, conn code)
+  std::vector<int> fill(pat.rowptr.begin(), pat.rowptr.end() - 1);
+  for (int i = 0; i < n; ++i) ent[fill[i]++] = {i, -1};
+  for (int f = 0; f < nconn; ++f) {
+    int a = N[2 * f], b = N[2 * f + 1];
+    ent[fill[a]++] = {b, 2 * f};
+    ent[fill[b]++] = {a, 2 * f + 1};
+  }
+  pat.col.resize(nnz);
+  entry_conn.resize(nnz);
+  pat.diag.resize(n);
+  for (int i = 0; i < n; ++i) {
+    auto b0 = ent.begin() + pat.rowptr[i], b1 = ent.begin() + pat.rowptr[i + 1];
+    std::stable_sort(b0, b1, [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+      return x.first < y.first;
+    });
+    for (int k = pat.rowptr[i]; k < pat.rowptr[i + 1]; ++k) {
+      pat.col[k] = ent[k].first;
+      entry_conn[k] = ent[k].second;
+      if (ent[k].first == i) pat.diag[i] = k;
+    }
+  }
+}
+
+// Greedy strength-based aggregation (the classic three-phase scheme of
+// smoothed-aggregation AMG: seed aggregates from nodes whose strong
+// neighbourhood is untouched, attach the rest to their strongest aggregated
+// neighbour, then sweep up leftovers).
+static int aggregate_greedy(const HostPattern& P, const std::vector<double>& w, double theta,
+                            std::vector<int>& agg) {
+  const int n = P.n;
+  std::vector<double> wmax(n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+      if (P.col[k] != i) wmax[i] = std::max(wmax[i], w[k]);
+  auto strong = [&](int i, int k) {
+    int j = P.col[k];
+    if (j == i) return false;
+    double t = theta * std::sqrt(wmax[i] * wmax[j]);
+    return w[k] > 0.0 && w[k] >= t;
+  };
+  agg.assign(n, -1);
+  int na = 0;
+  // phase 1
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    bool free_nbhd = true;
+    int nstrong = 0;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+      if (strong(i, k)) {
+        ++nstrong;
+        if (agg[P.col[k]] >= 0) { free_nbhd = false; break; }
+      }
+    if (!free_nbhd || nstrong == 0) continue;
+    agg[i] = na;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+      if (strong(i, k)) agg[P.col[k]] = na;
+    ++na;
+  }
+  // phase 2: attach to the strongest aggregated strong neighbour
+  std::vector<int> snap = agg;
+  for (int i = 0; i < n; ++i) {
+    if (snap[i] >= 0) continue;
+    double best = -1.0;
+    int ba = -1;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+      int j = P.col[k];
+      if (strong(i, k) && snap[j] >= 0 && w[k] > best) { best = w[k]; ba = snap[j]; }
+    }
+    if (ba >= 0) agg[i] = ba;
+  }
+  // phase 3: leftovers form aggregates with their unaggregated strong neighbours;
+  // a node with no strong neighbour joins its strongest neighbour's aggregate or
+  // stays alone.
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    int nstrong = 0;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+      if (strong(i, k)) ++nstrong;
+    if (nstrong == 0) {
+      double best = -1.0;
+      int ba = -1;
+      for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+        int j = P.col[k];
+        if (j != i && agg[j] >= 0 && w[k] > best) { best = w[k]; ba = agg[j]; }
+      }
+      agg[i] = ba >= 0 ? ba : na++;
+      continue;
+    }
+    agg[i] = na;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k)
+      if (strong(i, k) && agg[P.col[k]] < 0) agg[P.col[k]] = na;
+    ++na;
+  }
+  return na;
+}
+
+// Pairwise matching by strongest mutual connection (one pass halves the
+// number of unknowns at most); repeated passes give aggregates of 4, 8, ...
+static int aggregate_pairwise(const HostPattern& P, const std::vector<double>& w, std::vector<int>& agg) {
+  const int n = P.n;
+  agg.assign(n, -1);
+  int na = 0;
+  for (int i = 0; i < n; ++i) {
+    if (agg[i] >= 0) continue;
+    double best = 0.0;
+    int bj = -1;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+      int j = P.col[k];
+      if (j != i && agg[j] < 0 && w[k] > best) { best = w[k]; bj = j; }
+    }
+    agg[i] = na;
+    if (bj >= 0) agg[bj] = na;
+    ++na;
+  }
+  return na;
+}
+
+// Coarse pattern, Galerkin gather map (for piecewise-constant prolongation)
+// and coarse strength weights.
+static void coarsen(const HostPattern& P, const std::vector<double>& w, const std::vector<int>& agg, int nc,
+                    HostLevel& L, HostPattern& CP, std::vector<double>& cw) {
+  const int n = P.n;
+  L.agg = agg;
+  L.ncoarse = nc;
+  L.agg_ptr.assign(nc + 1, 0);
+  for (int i = 0; i < n; ++i) L.agg_ptr[agg[i] + 1]++;
+  for (int I = 0; I < nc; ++I) L.agg_ptr[I + 1] += L.agg_ptr[I];
+  L.agg_mem.resize(n);
+  {
+    std::vector<int> f(L.agg_ptr.begin(), L.agg_ptr.end() - 1);
+    for (int i = 0; i < n; ++i) L.agg_mem[f[agg[i]]++] = i;
+  }
+  CP.n = nc;
+  CP.rowptr.assign(nc + 1, 0);
+  std::vector<int> cols;
+  std::vector<int> mark(nc, -1);
+  std::vector<std::vector<int>> rows(nc);
+  for (int I = 0; I < nc; ++I) {
+    auto& r = rows[I];
+    r.push_back(I);
+    mark[I] = I;
+    for (int m = L.agg_ptr[I]; m < L.agg_ptr[I + 1]; ++m) {
+      int i = L.agg_mem[m];
+      for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+        int J = agg[P.col[k]];
+        if (mark[J] != I) { mark[J] = I; r.push_back(J); }
+      }
+    }
+    std::sort(r.begin(), r.end());
+    CP.rowptr[I + 1] = CP.rowptr[I] + (int)r.size();
+  }
+  int cnnz = CP.rowptr[nc];
+  CP.col.resize(cnnz);
+  CP.diag.resize(nc);
+  for (int I = 0; I < nc; ++I) {
+    std::copy(rows[I].begin(), rows[I].end(), CP.col.begin() + CP.rowptr[I]);
+    for (int K = CP.rowptr[I]; K < CP.rowptr[I + 1]; ++K)
+      if (CP.col[K] == I) CP.diag[I] = K;
+    std::vector<int>().swap(rows[I]);
+  }
+  // gather map
+  std::vector<int> dest(P.rowptr[n]);
+  L.gal_ptr.assign(cnnz + 1, 0);
+  for (int I = 0; I < nc; ++I)
+    for (int m = L.agg_ptr[I]; m < L.agg_ptr[I + 1]; ++m) {
+      int i = L.agg_mem[m];
+      for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+        int J = agg[P.col[k]];
+        auto b0 = CP.col.begin() + CP.rowptr[I], b1 = CP.col.begin() + CP.rowptr[I + 1];
+        int K = (int)(std::lower_bound(b0, b1, J) - CP.col.begin());
+        dest[k] = K;
+        L.gal_ptr[K + 1]++;
+      }
+    }
+  for (int K = 0; K < cnnz; ++K) L.gal_ptr[K + 1] += L.gal_ptr[K];
+  L.gal_src.resize(P.rowptr[n]);
+  cw.assign(cnnz, 0.0);
+  {
+    std::vector<int> f(L.gal_ptr.begin(), L.gal_ptr.end() - 1);
+    for (int I = 0; I < nc; ++I)
+      for (int m = L.agg_ptr[I]; m < L.agg_ptr[I + 1]; ++m) {
+        int i = L.agg_mem[m];
+        for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+          int K = dest[k];
+          L.gal_src[f[K]++] = k;
+          if (CP.col[K] != I && P.col[k] != i) cw[K] += w[k];
+        }
+      }
+  }
+}
+
+void build_aggregation(const HostPattern& fine, const std::vector<double>& w, const AggregationParams& prm,
+                       std::vector<HostLevel>& levels) {
+  levels.clear();
+  levels.emplace_back();
+  levels[0].pat = fine;
+  levels[0].w = w;
+  while (true) {
+    HostLevel& L = levels.back();
+    const int n = L.pat.n;
+    if (n <= prm.coarse_size || (int)levels.size() >= prm.max_levels) break;
+    std::vector<int> agg;
+    int nc = 0;
+    if (prm.pairwise_passes > 0) {
+      // repeated pairwise matching on successively coarser graphs
+      HostPattern P = L.pat;
+      std::vector<double> ww = L.w;
+      std::vector<int> total(n);
+      std::iota(total.begin(), total.end(), 0);
+      for (int pass = 0; pass < prm.pairwise_passes; ++pass) {
+        std::vector<int> a;
+        int m = aggregate_pairwise(P, ww, a);
+        for (int i = 0; i < n; ++i) total[i] = a[total[i]];
+        if (pass + 1 < prm.pairwise_passes) {
+          HostLevel tmp;
+          HostPattern CP;
+          std::vector<double> cw;
+          coarsen(P, ww, a, m, tmp, CP, cw);
+          P = std::move(CP);
+          ww = std::move(cw);
+        }
+        nc = m;
+      }
+      agg = std::move(total);
+    } else {
+      nc = aggregate_greedy(L.pat, L.w, prm.theta, agg);
+    }
+    if (nc <= 0 || (long long)nc * 100 > (long long)prm.min_coarse_reduction_pct * n) break;
+    HostPattern CP;
+    std::vector<double> cw;
+    coarsen(L.pat, L.w, agg, nc, L, CP, cw);
+    levels.emplace_back();
+    levels.back().pat = std::move(CP);
+    levels.back().w = std::move(cw);
+  }
+  // the last level carries no transfer
+  levels.back().agg.clear();
+  levels.back().ncoarse = 0;
+}
+
+}  // namespace pmx
