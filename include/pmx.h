@@ -39,6 +39,7 @@ typedef struct pmx_solver_options {
   int32_t pairwise_passes;
   int32_t reorthogonalize;
   int32_t verbose;
+  int32_t local_stage;      /* 0: cell block-Jacobi, 1: line block-tridiagonal solves */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {
@@ -112,6 +113,19 @@ int pmx_ow_export_blocks(pmx_ow* h, double* vals);
 /* Connection surface rates (cq_w, cq_o) per perforation of the last
  * assembly, into host out[2*nperf]. */
 int pmx_ow_connection_rates(pmx_ow* h, double* out);
+
+/* Lines of cells for the line local stage of CPR (host arrays:
+ * line_ptr[nlines+1], cells[line_ptr[nlines]]; consecutive cells of a line
+ * must be connected).  Without this call lines are detected from the
+ * transmissibilities.  No reference counterpart: the reference's local
+ * stages are block-Jacobi and ILU (linear_solvers.py:292-363). */
+int pmx_ow_set_lines(pmx_ow* h, int32_t nlines, const int32_t* line_ptr, const int32_t* cells);
+
+/* Kernel timing with HIP events on the handle's stream: enable (1) / disable
+ * (0, also clears); out4 (host) = {BCSR SpMV launches inside FGMRES, their
+ * total ms, preconditioner applications, their total ms}. */
+int pmx_ow_profile(pmx_ow* h, int32_t enable);
+int pmx_ow_profile_read(pmx_ow* h, double* out4);
 
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);

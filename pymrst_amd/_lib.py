@@ -29,6 +29,7 @@ class SolverOptions(ctypes.Structure):
         ("pairwise_passes", ctypes.c_int32),
         ("reorthogonalize", ctypes.c_int32),
         ("verbose", ctypes.c_int32),
+        ("local_stage", ctypes.c_int32),
     ]
 
 
@@ -66,6 +67,9 @@ SIGNATURES = {
     "pmx_ow_export_pattern": (ctypes.c_int, [_P, _P, _P]),
     "pmx_ow_export_blocks": (ctypes.c_int, [_P, _P]),
     "pmx_ow_connection_rates": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_set_lines": (ctypes.c_int, [_P, _I, _P, _P]),
+    "pmx_ow_profile": (ctypes.c_int, [_P, _I]),
+    "pmx_ow_profile_read": (ctypes.c_int, [_P, _P]),
     "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
 }
 

@@ -46,6 +46,7 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.pairwise_passes = o->pairwise_passes;
   r.reorthogonalize = o->reorthogonalize;
   r.verbose = o->verbose;
+  r.local_stage = o->local_stage;
   return r;
 }
 
@@ -71,6 +72,7 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->pairwise_passes = d.pairwise_passes;
   o->reorthogonalize = d.reorthogonalize;
   o->verbose = d.verbose;
+  o->local_stage = d.local_stage;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {
@@ -236,6 +238,28 @@ int pmx_ow_connection_rates(pmx_ow* h, double* out) {
       PMX_HIP(hipMemcpyAsync(out, m.rq.p, sizeof(double) * 2 * m.S.nperf, hipMemcpyDeviceToHost, m.stream));
       PMX_HIP(hipStreamSynchronize(m.stream));
     }
+  })
+}
+
+int pmx_ow_set_lines(pmx_ow* h, int32_t nlines, const int32_t* line_ptr, const int32_t* cells) {
+  PMX_TRY({
+    PMX_CHECK(h && line_ptr && cells && nlines >= 0, "null argument");
+    cpr_set_lines(h->m->cpr, nlines, line_ptr, cells);
+  })
+}
+
+int pmx_ow_profile(pmx_ow* h, int32_t enable) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    cpr_profile(h->m->cpr, enable != 0);
+  })
+}
+
+int pmx_ow_profile_read(pmx_ow* h, double* out4) {
+  PMX_TRY({
+    PMX_CHECK(h && out4, "null argument");
+    bind(h);
+    cpr_profile_read(h->m->cpr, out4);
   })
 }
 

@@ -39,8 +39,22 @@ struct BlockSystem {
   void set_wells(int nw_, const std::vector<int>& perf_ptr, const std::vector<int>& perf_cells, hipStream_t s);
 };
 
-// y = J x  (double)
-void system_matvec(const BlockSystem& S, const double* x, double* y, double* scratch_h, hipStream_t s);
+// Pairs of HIP events recorded around one kernel launch on its stream; the
+// elapsed times are summed when read.
+struct EventTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  void start(hipStream_t s);
+  void stop(hipStream_t s);
+  double total_ms(int* count);   // synchronises on the last event
+  void reset() { used = 0; }
+  ~EventTimer();
+};
+
+// y = J x  (double); the BCSR kernel launch is timed by t when t->on.
+void system_matvec(const BlockSystem& S, const double* x, double* y, double* scratch_h, hipStream_t s,
+                   EventTimer* t = nullptr);
 
 struct LinearSolverOptions {
   double rtol = 1e-8;
@@ -57,6 +71,7 @@ struct LinearSolverOptions {
   int pairwise_passes = 0;
   int reorthogonalize = 1;
   int verbose = 0;
+  int local_stage = 1;       // 0: cell block-Jacobi; 1: line block-tridiagonal solves
 };
 
 struct SolveInfo {
@@ -76,6 +91,9 @@ void cpr_destroy(CprSolver*);
 void cpr_build_hierarchy(CprSolver*, const BlockSystem& S, const std::vector<double>& weights,
                          const LinearSolverOptions& o, hipStream_t s);
 bool cpr_has_hierarchy(const CprSolver*);
+// Lines for the local stage (line_ptr[nlines+1], cells); without them lines
+// are detected from the strength weights.
+void cpr_set_lines(CprSolver*, int nlines, const int* ptr, const int* cells);
 // Numeric setup for the current values of S (pressure weights: w_dev[B]).
 void cpr_setup(CprSolver*, const BlockSystem& S, const double* w_dev, const LinearSolverOptions& o,
                hipStream_t s);
@@ -85,6 +103,10 @@ int cpr_solve(CprSolver*, const BlockSystem& S, const double* b, double* x, cons
 // z = M^{-1} r (one preconditioner application), exposed for tests.
 void cpr_apply(CprSolver*, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
                hipStream_t s);
+
+void cpr_profile(CprSolver*, bool on);
+// out[0..3] = {spmv launches, spmv ms, preconditioner applications, preconditioner ms}
+void cpr_profile_read(CprSolver*, double* out);
 
 // Helpers used by several sources.
 int grid_for(size_t threads, int block = 256);

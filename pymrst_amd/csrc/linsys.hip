@@ -204,13 +204,44 @@ __global__ void k_perf_add(int npc, const int* __restrict__ pc_cell, const int* 
   for (int r = 0; r < B; ++r) y[(size_t)c * B + r] += acc[r];
 }
 
+EventTimer::~EventTimer() {
+  for (auto e : ev) (void)hipEventDestroy(e);
+}
+void EventTimer::start(hipStream_t s) {
+  if (!on) return;
+  while (ev.size() < used + 2) {
+    hipEvent_t e;
+    PMX_HIP(hipEventCreate(&e));
+    ev.push_back(e);
+  }
+  PMX_HIP(hipEventRecord(ev[used], s));
+}
+void EventTimer::stop(hipStream_t s) {
+  if (!on) return;
+  PMX_HIP(hipEventRecord(ev[used + 1], s));
+  used += 2;
+}
+double EventTimer::total_ms(int* count) {
+  double t = 0.0;
+  if (used) PMX_HIP(hipEventSynchronize(ev[used - 1]));
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    float ms = 0.f;
+    PMX_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+    t += ms;
+  }
+  if (count) *count = (int)(used / 2);
+  return t;
+}
+
 template <int B, int M, int X>
-static void matvec_t(const BlockSystem& S, const double* x, double* y, double* h, hipStream_t s) {
+static void matvec_t(const BlockSystem& S, const double* x, double* y, double* h, hipStream_t s, EventTimer* t) {
   const size_t ncB = (size_t)S.nc * B;
   constexpr int LPR = 8;
   if (S.nc > 0) {
+    if (t) t->start(s);
     hipLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
                        S.rowptr.p, S.col.p, S.val.p, x, y);
+    if (t) t->stop(s);
     launch_check();
   }
   if (S.nw > 0) {
@@ -241,8 +272,8 @@ static void matvec_t(const BlockSystem& S, const double* x, double* y, double* h
     }                                                                             \
   } while (0)
 
-void system_matvec(const BlockSystem& S, const double* x, double* y, double* h, hipStream_t s) {
-  PMX_DISPATCH(S, (matvec_t<B_, M_, X_>(S, x, y, h, s)));
+void system_matvec(const BlockSystem& S, const double* x, double* y, double* h, hipStream_t s, EventTimer* t) {
+  PMX_DISPATCH(S, (matvec_t<B_, M_, X_>(S, x, y, h, s, t)));
 }
 
 // ---------------------------------------------------------------------------
@@ -569,6 +600,191 @@ __global__ void k_cpr_mixfix(int npc, const int* __restrict__ pc_cell, const int
 }
 
 // ---------------------------------------------------------------------------
+// Line local stage: exact block-tridiagonal solves along lines of strongly
+// coupled cells (one thread per line; position-major tables keep the lanes
+// of a wave on neighbouring cells).
+// ---------------------------------------------------------------------------
+template <int B>
+__device__ __forceinline__ void blk_load(const double* p, double a[B][B]) {
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) a[i][j] = p[i * B + j];
+}
+
+template <int B, class T>
+__device__ __forceinline__ void blk_store(T* p, const double a[B][B]) {
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) p[i * B + j] = (T)a[i][j];
+}
+
+// Block LU along each line: D'_k = D_k - L_k D'_{k-1}^{-1} U_{k-1}.
+template <int B, class T>
+__global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __restrict__ cell,
+                              const int* __restrict__ lo, const int* __restrict__ up, const int* __restrict__ diag,
+                              const double* __restrict__ val, T* __restrict__ inv, T* __restrict__ Lo,
+                              T* __restrict__ Up) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  const int L = len[l];
+  double Pinv[B][B], Uprev[B][B];
+  for (int k = 0; k < L; ++k) {
+    const size_t at = (size_t)k * nl + l;
+    const int c = cell[at];
+    double D[B][B], Di[B][B];
+    blk_load<B>(val + (size_t)diag[c] * B * B, D);
+    if (k > 0) {
+      double Lk[B][B], t[B][B];
+      blk_load<B>(val + (size_t)lo[at] * B * B, Lk);
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < B; ++q) s += Lk[i][q] * Pinv[q][j];
+          t[i][j] = s;
+        }
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < B; ++q) s += t[i][q] * Uprev[q][j];
+          D[i][j] -= s;
+        }
+      blk_store<B, T>(Lo + at * B * B, Lk);
+    }
+    small_inverse<B>(D, Di);
+    blk_store<B, T>(inv + at * B * B, Di);
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) Pinv[i][j] = Di[i][j];
+    if (k + 1 < L) {
+      blk_load<B>(val + (size_t)up[at] * B * B, Uprev);
+      blk_store<B, T>(Up + at * B * B, Uprev);
+    }
+  }
+}
+
+// r2 = r - J P xp on the cell rows (LPR lanes per row).
+template <int B, class T, int LPR>
+__global__ __launch_bounds__(256) void k_cpr_residual(int nc, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col, const T* __restrict__ Jp,
+                                                      const T* __restrict__ xp, const double* __restrict__ r,
+                                                      T* __restrict__ r2) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = t / LPR, lane = t % LPR;
+  double acc[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) acc[a] = 0.0;
+  if (row < nc) {
+    const int e = rowptr[row + 1];
+    for (int k = rowptr[row] + lane; k < e; k += LPR) {
+      const double xc = (double)xp[col[k]];
+#pragma unroll
+      for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)k * B + a], xc, acc[a]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < B; ++a) acc[a] = wave_sum(acc[a], LPR);
+  if (row < nc && lane == 0) {
+#pragma unroll
+    for (int a = 0; a < B; ++a) r2[(size_t)row * B + a] = (T)(r[(size_t)row * B + a] - acc[a]);
+  }
+}
+
+// Mixture coupling of stage 2 on perforated cells: r2_c -= sum_j Um_j hp_wj.
+template <int B, int X, class T>
+__global__ void k_cpr_mixfix_r2(int npc, const int* __restrict__ pc_cell, const int* __restrict__ pc_ptr,
+                                const int* __restrict__ pc_perf, const int* __restrict__ perf_well,
+                                const double* __restrict__ Um, const double* __restrict__ hp, T* __restrict__ r2) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= npc) return;
+  const int c = pc_cell[u];
+  for (int t = pc_ptr[u]; t < pc_ptr[u + 1]; ++t) {
+    const int j = pc_perf[t], w = perf_well[j];
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+      double s = 0.0;
+#pragma unroll
+      for (int x = 0; x < X; ++x) s += Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
+      r2[(size_t)c * B + a] = (T)((double)r2[(size_t)c * B + a] - s);
+    }
+  }
+}
+
+// z = P xp + (block-tridiagonal line solve of r2).
+template <int B, class T>
+__global__ void k_line_solve(int nl, const int* __restrict__ len, const int* __restrict__ cell,
+                             const T* __restrict__ inv, const T* __restrict__ Lo, const T* __restrict__ Up,
+                             const T* __restrict__ r2, const T* __restrict__ xp, double* __restrict__ z) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  const int L = len[l];
+  double y[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) y[a] = 0.0;
+  for (int k = 0; k < L; ++k) {
+    const size_t at = (size_t)k * nl + l;
+    const int c = cell[at];
+    double rhs[B];
+#pragma unroll
+    for (int a = 0; a < B; ++a) rhs[a] = (double)r2[(size_t)c * B + a];
+    if (k > 0) {
+#pragma unroll
+      for (int a = 0; a < B; ++a)
+#pragma unroll
+        for (int q = 0; q < B; ++q) rhs[a] -= (double)Lo[at * B * B + a * B + q] * y[q];
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < B; ++q) s += (double)inv[at * B * B + a * B + q] * rhs[q];
+      z[(size_t)c * B + a] = s;
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) y[a] = z[(size_t)c * B + a];
+  }
+  // backward: x_k = y_k - D'_k^{-1} U_k x_{k+1};  z = x + P xp
+  double xn[B];
+  for (int k = L - 1; k >= 0; --k) {
+    const size_t at = (size_t)k * nl + l;
+    const int c = cell[at];
+    double x[B];
+#pragma unroll
+    for (int a = 0; a < B; ++a) x[a] = z[(size_t)c * B + a];
+    if (k + 1 < L) {
+      double u[B];
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) s += (double)Up[at * B * B + a * B + q] * xn[q];
+        u[a] = s;
+      }
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) s += (double)inv[at * B * B + a * B + q] * u[q];
+        x[a] -= s;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+      xn[a] = x[a];
+      z[(size_t)c * B + a] = x[a] + (a == 0 ? (double)xp[c] : 0.0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Aggregation AMG kernels (scalar CSR, value type T)
 // ---------------------------------------------------------------------------
 template <class T>
@@ -714,11 +930,39 @@ class CprSolver {
   size_t n_alloc = 0, ld = 0;
   int prec_setup = 0;
   const double* wdev = nullptr;       // pressure weights of the current setup
+  EventTimer spmv_t, prec_t;
+  // line local stage: position-major line tables and block-tridiagonal factors
+  int nlines = 0, maxlen = 0, longest = 0;
+  std::vector<int> user_line_ptr, user_line_cells;   // lines given by the caller
+  DArr<int> ln_len, ln_cell, ln_lo, ln_up;
+  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64;
+  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32;
 };
+
+void cpr_profile(CprSolver* C, bool on) {
+  C->spmv_t.on = on;
+  C->prec_t.on = on;
+  C->spmv_t.reset();
+  C->prec_t.reset();
+}
+
+void cpr_profile_read(CprSolver* C, double* out) {
+  int n1 = 0, n2 = 0;
+  out[1] = C->spmv_t.total_ms(&n1);
+  out[3] = C->prec_t.total_ms(&n2);
+  out[0] = n1;
+  out[2] = n2;
+}
 
 CprSolver* cpr_create() { return new CprSolver(); }
 void cpr_destroy(CprSolver* c) { delete c; }
 bool cpr_has_hierarchy(const CprSolver* c) { return c && c->built; }
+
+void cpr_set_lines(CprSolver* C, int nlines, const int* ptr, const int* cells) {
+  C->user_line_ptr.assign(ptr, ptr + nlines + 1);
+  C->user_line_cells.assign(cells, cells + ptr[nlines]);
+  C->built = false;   // rebuild the static structures with these lines
+}
 
 void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<double>& weights,
                          const LinearSolverOptions& o, hipStream_t s) {
@@ -758,6 +1002,19 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
     }
     C->L.push_back(std::move(lv));
   }
+  // lines for the local stage, from the same strength weights
+  HostLines HL;
+  if (!C->user_line_ptr.empty())
+    lines_from_list(fine, (int)C->user_line_ptr.size() - 1, C->user_line_ptr.data(), C->user_line_cells.data(), HL);
+  else
+    build_lines(fine, weights, HL);
+  C->nlines = HL.nlines;
+  C->maxlen = HL.maxlen;
+  C->longest = HL.nlines ? HL.len[0] : 0;
+  C->ln_len.upload(HL.len, s);
+  C->ln_cell.upload(HL.cell, s);
+  C->ln_lo.upload(HL.lo, s);
+  C->ln_up.upload(HL.up, s);
   C->coarse_n = C->L.back()->n;
   PMX_CHECK(C->coarse_n <= 4096, "coarsest AMG level is too large for the dense solve");
   C->dense.alloc((size_t)C->coarse_n * C->coarse_n);
@@ -767,7 +1024,7 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   if (o.verbose) {
     fprintf(stderr, "[pmx] AMG hierarchy:");
     for (auto& lv : C->L) fprintf(stderr, " %d(%d)", lv->n, lv->nnz);
-    fprintf(stderr, "\n");
+    fprintf(stderr, "\n[pmx] lines: %d, longest %d\n", C->nlines, C->longest);
   }
 }
 
@@ -785,6 +1042,10 @@ struct Lv<double> {
   static DArr<double>& r(AmgLevel& l) { return l.r64; }
   static DArr<double>& Jp(CprSolver& c) { return c.Jp64; }
   static DArr<double>& inv(CprSolver& c) { return c.inv64; }
+  static DArr<double>& lf_inv(CprSolver& c) { return c.lf_inv64; }
+  static DArr<double>& lf_lo(CprSolver& c) { return c.lf_lo64; }
+  static DArr<double>& lf_up(CprSolver& c) { return c.lf_up64; }
+  static DArr<double>& r2(CprSolver& c) { return c.r2_64; }
 };
 template <>
 struct Lv<float> {
@@ -796,6 +1057,10 @@ struct Lv<float> {
   static DArr<float>& r(AmgLevel& l) { return l.r32; }
   static DArr<float>& Jp(CprSolver& c) { return c.Jp32; }
   static DArr<float>& inv(CprSolver& c) { return c.inv32; }
+  static DArr<float>& lf_inv(CprSolver& c) { return c.lf_inv32; }
+  static DArr<float>& lf_lo(CprSolver& c) { return c.lf_lo32; }
+  static DArr<float>& lf_up(CprSolver& c) { return c.lf_up32; }
+  static DArr<float>& r2(CprSolver& c) { return c.r2_32; }
 };
 
 template <int B, int M, int X, class T>
@@ -823,6 +1088,17 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
   hipLaunchKernelGGL((k_block_inverse<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, S.diag.p, S.val.p,
                      inv.p);
   launch_check();
+  if (o.local_stage == 1 && C->nlines > 0) {
+    const size_t m = (size_t)C->maxlen * C->nlines * B * B;
+    V::lf_inv(*C).alloc(m);
+    V::lf_lo(*C).alloc(m);
+    V::lf_up(*C).alloc(m);
+    V::r2(*C).alloc((size_t)S.nc * B);
+    hipLaunchKernelGGL((k_line_factor<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
+                       C->ln_len.p, C->ln_cell.p, C->ln_lo.p, C->ln_up.p, S.diag.p, S.val.p, V::lf_inv(*C).p,
+                       V::lf_lo(*C).p, V::lf_up(*C).p);
+    launch_check();
+  }
   if (S.nw > 0) {
     C->tail_inv.alloc((size_t)S.nw * M * M);
     C->hp.alloc((size_t)S.nw * X);
@@ -907,6 +1183,10 @@ static void vcycle(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStre
 }
 
 template <int B, int M, int X, class T>
+static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
+                   hipStream_t s);
+
+template <int B, int M, int X, class T>
 static void apply_t(CprSolver* C, const BlockSystem& S, const double* r, double* z, const double* wdev,
                     const LinearSolverOptions& o, hipStream_t s) {
   using V = Lv<T>;
@@ -921,38 +1201,57 @@ static void apply_t(CprSolver* C, const BlockSystem& S, const double* r, double*
     // further cycles on the pressure residual: b <- b - A x, correction added
     throw std::invalid_argument("amg_cycles > 1 is not supported");
   }
-  // stage 2: local block-Jacobi on r - J P xp
-  hipLaunchKernelGGL((k_cpr_stage2<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, r, V::inv(*C).p, z);
-  launch_check();
+  stage2<B, M, X, T>(C, S, r, z, o, s);
+}
+
+// Stage 2: z = P xp + local(r - J P xp) with the local stage chosen by the
+// options (cell block-Jacobi, or line block-tridiagonal solves); the well
+// tail is solved exactly per well.
+template <int B, int M, int X, class T>
+static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
+                   hipStream_t s) {
+  using V = Lv<T>;
+  AmgLevel& f = *C->L[0];
+  constexpr int LPR = 8;
+  const size_t ncB = (size_t)S.nc * B;
   if (S.nw > 0) {
     hipLaunchKernelGGL((k_cpr_wells<B, M, X, T>), dim3(S.nw), dim3(64), 0, s, S.nw, S.well_ptr.p, S.perf_cell.p,
                        S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, r, ncB, z, C->hp.p);
     launch_check();
-    if (S.has_mix && S.npc > 0) {
-      hipLaunchKernelGGL((k_cpr_mixfix<B, X, T>), dim3(grid_for(S.npc)), dim3(256), 0, s, S.npc, S.pc_cell.p,
-                         S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::inv(*C).p, z);
+  }
+  if (o.local_stage == 1 && C->nlines > 0) {
+    hipLaunchKernelGGL((k_cpr_residual<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                       S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, r, V::r2(*C).p);
+    launch_check();
+    if (S.nw > 0 && S.has_mix && S.npc > 0) {
+      hipLaunchKernelGGL((k_cpr_mixfix_r2<B, X, T>), dim3(grid_for(S.npc)), dim3(256), 0, s, S.npc, S.pc_cell.p,
+                         S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p);
       launch_check();
     }
+    hipLaunchKernelGGL((k_line_solve<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
+                       C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p,
+                       V::x(f).p, z);
+    launch_check();
+    return;
+  }
+  hipLaunchKernelGGL((k_cpr_stage2<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, r, V::inv(*C).p, z);
+  launch_check();
+  if (S.nw > 0 && S.has_mix && S.npc > 0) {
+    hipLaunchKernelGGL((k_cpr_mixfix<B, X, T>), dim3(grid_for(S.npc)), dim3(256), 0, s, S.npc, S.pc_cell.p,
+                       S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::inv(*C).p, z);
+    launch_check();
   }
 }
 
 // Preconditioner without the pressure stage: block-Jacobi + tail only.
 template <int B, int M, int X, class T>
-static void apply_local_t(CprSolver* C, const BlockSystem& S, const double* r, double* z, hipStream_t s) {
+static void apply_local_t(CprSolver* C, const BlockSystem& S, const double* r, double* z,
+                          const LinearSolverOptions& o, hipStream_t s) {
   using V = Lv<T>;
   AmgLevel& f = *C->L[0];
-  constexpr int LPR = 8;
-  const size_t ncB = (size_t)S.nc * B;
   PMX_HIP(hipMemsetAsync(V::x(f).p, 0, sizeof(T) * f.n, s));
-  hipLaunchKernelGGL((k_cpr_stage2<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, r, V::inv(*C).p, z);
-  launch_check();
-  if (S.nw > 0) {
-    hipLaunchKernelGGL((k_cpr_wells<B, M, X, T>), dim3(S.nw), dim3(64), 0, s, S.nw, S.well_ptr.p, S.perf_cell.p,
-                       S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, r, ncB, z, C->hp.p);
-    launch_check();
-  }
+  stage2<B, M, X, T>(C, S, r, z, o, s);
 }
 
 void cpr_apply(CprSolver* C, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
@@ -961,10 +1260,10 @@ void cpr_apply(CprSolver* C, const BlockSystem& S, const double* r, double* z, c
   PMX_CHECK(C->wdev != nullptr, "preconditioner weights unset");
   if (o.precision == 32) {
     if (o.use_pressure_stage) PMX_DISPATCH(S, (apply_t<B_, M_, X_, float>(C, S, r, z, C->wdev, o, s)));
-    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, float>(C, S, r, z, s)));
+    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, float>(C, S, r, z, o, s)));
   } else {
     if (o.use_pressure_stage) PMX_DISPATCH(S, (apply_t<B_, M_, X_, double>(C, S, r, z, C->wdev, o, s)));
-    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, double>(C, S, r, z, s)));
+    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, double>(C, S, r, z, o, s)));
   }
 }
 
@@ -1023,8 +1322,10 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       double* vj = C->V.p + (size_t)j * ld;
       double* zj = C->Z.p + (size_t)j * ld;
       double* w = C->V.p + (size_t)(j + 1) * ld;
+      C->prec_t.start(s);
       cpr_apply(C, S, vj, zj, o, s);
-      system_matvec(S, zj, w, hscr, s);
+      C->prec_t.stop(s);
+      system_matvec(S, zj, w, hscr, s, &C->spmv_t);
       // CGS pass 1: h = V^T w
       const int nd = j + 2;
       hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w,

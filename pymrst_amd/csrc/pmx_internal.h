@@ -89,4 +89,14 @@ struct AggregationParams {
 void build_aggregation(const HostPattern& fine, const std::vector<double>& w,
                        const AggregationParams& prm, std::vector<HostLevel>& levels);
 
+// Lines of strongly coupled cells, stored position-major (entry k*nlines + l
+// is the k-th cell of line l, -1 past its end); lo/up are the BCSR entries of
+// the blocks (c_k, c_{k-1}) and (c_k, c_{k+1}).
+struct HostLines {
+  int nlines = 0, maxlen = 0;
+  std::vector<int> len, cell, lo, up;
+};
+void build_lines(const HostPattern& P, const std::vector<double>& w, HostLines& L);
+void lines_from_list(const HostPattern& P, int nlines, const int* ptr, const int* cells, HostLines& L);
+
 }  // namespace pmx

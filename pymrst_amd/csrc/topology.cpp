@@ -269,3 +269,132 @@ void build_aggregation(const HostPattern& fine, const std::vector<double>& w, co
 }
 
 }  // namespace pmx
+
+namespace pmx {
+
+// Lines of strongly coupled cells: each cell links to its (at most) two
+// strongest neighbours; a link is kept when it is mutual, which leaves
+// paths and cycles; cycles are cut once.  On layered Cartesian grids with
+// thin cells these are the vertical columns.
+void build_lines(const HostPattern& P, const std::vector<double>& w, HostLines& L) {
+  const int n = P.n;
+  std::vector<int> s1(n, -1), s2(n, -1);
+  for (int i = 0; i < n; ++i) {
+    double b1 = 0.0, b2 = 0.0;
+    for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
+      const int j = P.col[k];
+      if (j == i || !(w[k] > 0.0)) continue;
+      if (w[k] > b1) {
+        b2 = b1; s2[i] = s1[i];
+        b1 = w[k]; s1[i] = j;
+      } else if (w[k] > b2) {
+        b2 = w[k]; s2[i] = j;
+      }
+    }
+  }
+  auto top2 = [&](int i, int j) { return s1[i] == j || s2[i] == j; };
+  std::vector<int> nb0(n, -1), nb1(n, -1);
+  auto link = [&](int i, int j) {
+    if (nb0[i] < 0) nb0[i] = j; else nb1[i] = j;
+  };
+  for (int i = 0; i < n; ++i)
+    for (int j : {s1[i], s2[i]})
+      if (j > i && top2(j, i)) { link(i, j); link(j, i); }
+  std::vector<char> seen(n, 0);
+  std::vector<std::vector<int>> lines;
+  auto walk = [&](int start) {
+    std::vector<int> line;
+    int prev = -1, cur = start;
+    while (cur >= 0 && !seen[cur]) {
+      seen[cur] = 1;
+      line.push_back(cur);
+      const int nxt = nb0[cur] != prev ? nb0[cur] : nb1[cur];
+      prev = cur;
+      cur = nxt;
+    }
+    return line;
+  };
+  for (int i = 0; i < n; ++i)   // path ends first
+    if (!seen[i] && (nb0[i] < 0 || nb1[i] < 0)) lines.push_back(walk(i));
+  for (int i = 0; i < n; ++i)   // what is left lies on cycles: cut at i
+    if (!seen[i]) lines.push_back(walk(i));
+  // ascending first cell, then stable by descending length
+  std::stable_sort(lines.begin(), lines.end(),
+                   [](const std::vector<int>& a, const std::vector<int>& b) { return a[0] < b[0]; });
+  std::stable_sort(lines.begin(), lines.end(),
+                   [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
+  const int nl = (int)lines.size();
+  const int maxlen = nl ? (int)lines[0].size() : 0;
+  L.nlines = nl;
+  L.maxlen = maxlen;
+  L.len.resize(nl);
+  L.cell.assign((size_t)maxlen * nl, -1);
+  L.lo.assign((size_t)maxlen * nl, -1);
+  L.up.assign((size_t)maxlen * nl, -1);
+  auto entry = [&](int r, int c) {
+    auto b0 = P.col.begin() + P.rowptr[r], b1 = P.col.begin() + P.rowptr[r + 1];
+    auto it = std::lower_bound(b0, b1, c);
+    PMX_CHECK(it != b1 && *it == c, "line link outside the pattern");
+    return (int)(it - P.col.begin());
+  };
+  for (int l = 0; l < nl; ++l) {
+    const auto& ln = lines[l];
+    L.len[l] = (int)ln.size();
+    for (size_t k = 0; k < ln.size(); ++k) {
+      const size_t at = k * nl + l;
+      L.cell[at] = ln[k];
+      if (k > 0) L.lo[at] = entry(ln[k], ln[k - 1]);
+      if (k + 1 < ln.size()) L.up[at] = entry(ln[k], ln[k + 1]);
+    }
+  }
+}
+
+}  // namespace pmx
+
+namespace pmx {
+
+// Lines given by the caller (line_ptr[nlines+1], cells[]), e.g. the columns
+// of a Cartesian grid; consecutive cells of a line must be connected.
+void lines_from_list(const HostPattern& P, int nlines, const int* ptr, const int* cells, HostLines& L) {
+  std::vector<std::vector<int>> lines(nlines);
+  std::vector<char> seen(P.n, 0);
+  for (int l = 0; l < nlines; ++l) {
+    PMX_CHECK(ptr[l + 1] > ptr[l], "empty line");
+    for (int k = ptr[l]; k < ptr[l + 1]; ++k) {
+      const int c = cells[k];
+      PMX_CHECK(c >= 0 && c < P.n && !seen[c], "line cells must be distinct cells of the grid");
+      seen[c] = 1;
+      lines[l].push_back(c);
+    }
+  }
+  for (int i = 0; i < P.n; ++i)
+    if (!seen[i]) lines.push_back(std::vector<int>{i});
+  std::stable_sort(lines.begin(), lines.end(),
+                   [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
+  const int nl = (int)lines.size();
+  const int maxlen = nl ? (int)lines[0].size() : 0;
+  L.nlines = nl;
+  L.maxlen = maxlen;
+  L.len.resize(nl);
+  L.cell.assign((size_t)maxlen * nl, -1);
+  L.lo.assign((size_t)maxlen * nl, -1);
+  L.up.assign((size_t)maxlen * nl, -1);
+  auto entry = [&](int r, int c) {
+    auto b0 = P.col.begin() + P.rowptr[r], b1 = P.col.begin() + P.rowptr[r + 1];
+    auto it = std::lower_bound(b0, b1, c);
+    PMX_CHECK(it != b1 && *it == c, "consecutive line cells are not connected");
+    return (int)(it - P.col.begin());
+  };
+  for (int l = 0; l < nl; ++l) {
+    const auto& ln = lines[l];
+    L.len[l] = (int)ln.size();
+    for (size_t k = 0; k < ln.size(); ++k) {
+      const size_t at = k * nl + l;
+      L.cell[at] = ln[k];
+      if (k > 0) L.lo[at] = entry(ln[k], ln[k - 1]);
+      if (k + 1 < ln.size()) L.up[at] = entry(ln[k], ln[k + 1]);
+    }
+  }
+}
+
+}  // namespace pmx

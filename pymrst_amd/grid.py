@@ -149,6 +149,25 @@ def tpfa_connections(g: Grid, t_half: np.ndarray):
     return g.neighbors[internal].copy(), T_all[internal].copy(), T_all, internal
 
 
+def cartesian_lines(g: Grid, N: np.ndarray, T: np.ndarray):
+    """Columns of a Cartesian grid along the axis with the largest total
+    transmissibility, as (line_ptr, cells) for the line local stage of CPR.
+    Lines are ordered by their first cell, so neighbouring lines hold
+    neighbouring cells."""
+    nx, ny, nz = g.cartDims
+    strides = (1, nx, nx * ny)
+    d = N[:, 1] - N[:, 0]
+    totals = [float(T[d == s].sum()) for s in strides]
+    axis = int(np.argmax(totals))
+    n = (nx, ny, nz)[axis]
+    idx = np.arange(g.num_cells).reshape((nz, ny, nx))   # [k, j, i]
+    cols = np.moveaxis(idx, 2 - axis, -1).reshape(-1, n)  # one row per line, ascending along the axis
+    order = np.argsort(cols[:, 0], kind="stable")
+    cols = cols[order]
+    ptr = np.arange(cols.shape[0] + 1, dtype=np.int32) * n
+    return axis, ptr, np.ascontiguousarray(cols.ravel(), dtype=np.int32)
+
+
 @dataclass
 class Well:
     cells: np.ndarray

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .grid import BARSA, Grid, Rock, Well, compute_trans, pore_volume, tpfa_connections
+from .grid import BARSA, Grid, Rock, Well, cartesian_lines, compute_trans, pore_volume, tpfa_connections
 
 
 @dataclass
@@ -110,6 +110,7 @@ class CPRSolver:
     pairwise_passes: int = 0
     reorthogonalize: bool = True
     verbose: int = 0
+    local_stage: str = "line"
     last_iterations: int = 0
     history: list = field(default_factory=list)
 
@@ -121,6 +122,7 @@ class CPRSolver:
         o.jacobi_omega, o.coarse_alpha, o.theta = self.jacobi_omega, self.coarse_alpha, self.theta
         o.coarse_size, o.pairwise_passes = self.coarse_size, self.pairwise_passes
         o.reorthogonalize, o.verbose = int(self.reorthogonalize), self.verbose
+        o.local_stage = {"block_jacobi": 0, "line": 1}[self.local_stage]
         return o
 
 
@@ -152,7 +154,8 @@ class TwoPhaseOilWaterModelAD:
         for w in self.wells:
             if w.type not in ("bhp", "rate"):
                 raise ValueError(f"unsupported well control '{w.type}'")
-        if N is None or T is None:
+        own_connections = N is None or T is None
+        if own_connections:
             N, T, _, _ = tpfa_connections(g, compute_trans(g, rock))
         self.N = np.ascontiguousarray(N, dtype=np.int32)
         self.T = np.ascontiguousarray(T, dtype=np.float64)
@@ -183,6 +186,11 @@ class TwoPhaseOilWaterModelAD:
                                                     _lib.ptr(self.T), _lib.ptr(gdz), _lib.ptr(pv)),
                        "pmx_ow_set_topology")
             self._set_wells()
+            if own_connections and getattr(g, "cartDims", None) is not None:
+                axis, ptr, cells = cartesian_lines(g, self.N, self.T)
+                _lib.check(self.lib.pmx_ow_set_lines(h, ptr.size - 1, _lib.ptr(ptr), _lib.ptr(cells)),
+                           "pmx_ow_set_lines")
+                self.line_axis = axis
         n = 2 * self.nc + 3 * self.nw
         self._dx = torch.zeros(n, dtype=torch.float64, device=self.device)
 
@@ -212,6 +220,28 @@ class TwoPhaseOilWaterModelAD:
             except Exception:
                 pass
             self.h = None
+
+    def reset(self):
+        """Forget per-run state (the previous step's well cdp) so a new run
+        from an initial state behaves like a fresh model."""
+        self._last_cdp = None
+        self._cdp_dev.zero_()
+        self.stats = {"newton": 0, "linear": 0, "assemble_s": 0.0, "solve_s": 0.0}
+
+    def profile(self, enable: bool):
+        _lib.check(self.lib.pmx_ow_profile(self.h, int(enable)), "pmx_ow_profile")
+
+    def profile_read(self) -> dict:
+        out = np.zeros(4)
+        _lib.check(self.lib.pmx_ow_profile_read(self.h, _lib.ptr(out)), "pmx_ow_profile_read")
+        return {"spmv_count": int(out[0]), "spmv_ms": float(out[1]), "prec_count": int(out[2]),
+                "prec_ms": float(out[3])}
+
+    @property
+    def nnzb(self) -> int:
+        v = ctypes.c_int32()
+        _lib.check(self.lib.pmx_ow_nnzb(self.h, ctypes.byref(v)), "pmx_ow_nnzb")
+        return int(v.value)
 
     # -- reference attribute contract ------------------------------------
     @property

