@@ -38,17 +38,25 @@ def run_form(case, precision, steps, warmup, rtol):
     import torch
     from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
     from pymrst_amd.workloads import build_model
-    solver = CPRSolver(rtol=rtol, precision=precision)
+    solver = CPRSolver(rtol=rtol, precision=precision, verbose=1)
+    t_setup = time.perf_counter()
     model, s0 = build_model(case, solver)
+    print(f"[bench] FP{precision}: model set up in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr,
+          flush=True)
     dts = case.dts
+
+    def progress(event, payload):
+        if event in ("step_accepted", "step_rejected", "timestep_cut"):
+            print(f"[bench] FP{precision} {event} {payload}", file=sys.stderr, flush=True)
+
     if warmup > 0:
-        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True)
+        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True, observer=progress)
     model.reset()
     solver.history.clear()
     model.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    states, its = simulate_schedule_ad(s0, model, dts[:steps], keep_device=True)
+    states, its = simulate_schedule_ad(s0, model, dts[:steps], keep_device=True, observer=progress)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     prof = model.profile_read()

@@ -28,7 +28,7 @@ typedef struct pmx_ow pmx_ow;
 typedef struct pmx_solver_options {
   double rtol;              /* ||b - J x|| <= rtol ||b|| */
   int32_t restart;
-  int32_t maxiter;
+  int32_t maxiter;          /* restart cycles, as scipy's gmres counts them */
   int32_t precision;        /* 64: FP64 preconditioner, 32: FP32 preconditioner */
   int32_t use_pressure_stage;
   int32_t pre_smooth, post_smooth;
@@ -129,6 +129,37 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4);
 
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);
+
+/* ---- incompressible TPFA pressure solve (incomp_tpfa_ad,
+ *      pymrst/modules/incomp/incomp_tpfa.py:252-389) and computeTrans
+ *      (pymrst/solvers/compute_trans.py:11-29) */
+typedef struct pmx_incomp pmx_incomp;
+
+int pmx_incomp_create(int32_t device, void* hip_stream, pmx_incomp** out);
+int pmx_incomp_destroy(pmx_incomp* h);
+
+/* Grid connectivity and geometry (host): face_pos[nc+1] and hf_face[nhf]
+ * are MRST's cells.facePos / cells.faces, face_nb[2*nf] its faces.neighbors
+ * (-1 outside), cell_centroids[3*nc], face_centroids[3*nf]. */
+int pmx_incomp_set_grid(pmx_incomp* h, int32_t nc, int32_t nf, const int32_t* face_pos, const int32_t* hf_face,
+                        const int32_t* face_nb, const double* cell_centroids, const double* face_centroids);
+
+/* computeTrans on the device: one-sided transmissibility per half-face into
+ * T[nhf] from face_normals[3*nf] and perm[nk*nc] (nk = 1, 3 or 6; all device). */
+int pmx_compute_trans(pmx_incomp* h, const double* face_normals, int32_t nk, const double* perm, double* T);
+
+/* Pressure solve.  Device: T[nhf] (one-sided transmissibilities), mob[nc]
+ * (total mobility), omega[nc] (mobility-weighted density), and the outputs
+ * p[nc], flux[nf], face_pressure[nf].  Host: gravity[3] (NULL = off),
+ * boundary conditions (bc_type 0 pressure, 1 flux), sources, wells
+ * (perf_ptr[nw+1], perf_cells, WI, wtype 0 bhp / 1 rate, wval, wdp = the
+ * connection pressure drops) and the outputs well_p[nw], well_flux[nperf]. */
+int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const double* omega, const double* gravity,
+                     int32_t nbc, const int32_t* bc_face, const int32_t* bc_type, const double* bc_value, int32_t nsrc,
+                     const int32_t* src_cell, const double* src_rate, int32_t nw, const int32_t* perf_ptr,
+                     const int32_t* perf_cells, const double* WI, const int32_t* wtype, const double* wval,
+                     const double* wdp, const pmx_solver_options* o, double* p, double* flux, double* face_pressure,
+                     double* well_p, double* well_flux, pmx_solve_info* info);
 
 #ifdef __cplusplus
 }

@@ -5,9 +5,15 @@
 #include <string>
 
 #include "../../include/pmx.h"
+#include "incomp.h"
 #include "ow.h"
 
 using namespace pmx;
+
+struct pmx_incomp {
+  int device = 0;
+  std::unique_ptr<IncompDev> m;
+};
 
 struct pmx_ow {
   int device = 0;
@@ -261,6 +267,99 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4) {
     bind(h);
     cpr_profile_read(h->m->cpr, out4);
   })
+}
+
+int pmx_incomp_create(int32_t device, void* hip_stream, pmx_incomp** out) {
+  PMX_TRY({
+    PMX_CHECK(out != nullptr, "null argument");
+    auto h = std::make_unique<pmx_incomp>();
+    h->device = device;
+    PMX_HIP(hipSetDevice(device));
+    h->m = std::make_unique<IncompDev>((hipStream_t)hip_stream);
+    *out = h.release();
+  })
+}
+
+int pmx_incomp_destroy(pmx_incomp* h) {
+  PMX_TRY({
+    if (h) {
+      PMX_HIP(hipSetDevice(h->device));
+      delete h;
+    }
+  })
+}
+
+int pmx_incomp_set_grid(pmx_incomp* h, int32_t nc, int32_t nf, const int32_t* face_pos, const int32_t* hf_face,
+                        const int32_t* face_nb, const double* cell_centroids, const double* face_centroids) {
+  PMX_TRY({
+    PMX_CHECK(h && face_pos && hf_face && face_nb && cell_centroids && face_centroids, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    h->m->set_grid(nc, nf, face_pos, hf_face, face_nb, cell_centroids, face_centroids);
+  })
+}
+
+int pmx_compute_trans(pmx_incomp* h, const double* face_normals, int32_t nk, const double* perm, double* T) {
+  PMX_TRY({
+    PMX_CHECK(h && face_normals && perm && T, "null argument");
+    PMX_CHECK(nk == 1 || nk == 3 || nk == 6, "permeability must have 1, 3 or 6 columns");
+    PMX_HIP(hipSetDevice(h->device));
+    h->m->compute_trans(face_normals, nk, perm, T);
+  })
+}
+
+int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const double* omega, const double* gravity,
+                     int32_t nbc, const int32_t* bc_face, const int32_t* bc_type, const double* bc_value, int32_t nsrc,
+                     const int32_t* src_cell, const double* src_rate, int32_t nw, const int32_t* perf_ptr,
+                     const int32_t* perf_cells, const double* WI, const int32_t* wtype, const double* wval,
+                     const double* wdp, const pmx_solver_options* o, double* p, double* flux, double* face_pressure,
+                     double* well_p, double* well_flux, pmx_solve_info* info) {
+  try {
+    PMX_CHECK(h && T && mob && omega && p && flux && face_pressure, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    IncompBC bc;
+    if (nbc > 0) {
+      bc.face.assign(bc_face, bc_face + nbc);
+      bc.type.assign(bc_type, bc_type + nbc);
+      bc.value.assign(bc_value, bc_value + nbc);
+    }
+    if (nsrc > 0) {
+      bc.src_cell.assign(src_cell, src_cell + nsrc);
+      bc.src_rate.assign(src_rate, src_rate + nsrc);
+    }
+    IncompWells W;
+    if (nw > 0) {
+      PMX_CHECK(perf_ptr && perf_cells && WI && wtype && wval && wdp && well_p && well_flux, "null well argument");
+      const int np = perf_ptr[nw];
+      W.perf_ptr.assign(perf_ptr, perf_ptr + nw + 1);
+      W.cells.assign(perf_cells, perf_cells + np);
+      W.WI.assign(WI, WI + np);
+      W.dp.assign(wdp, wdp + np);
+      W.type.assign(wtype, wtype + nw);
+      W.val.assign(wval, wval + nw);
+      for (int c : W.cells) PMX_CHECK(c >= 0 && c < h->m->nc, "well connection cell out of range");
+      for (int t : W.type) PMX_CHECK(t == 0 || t == 1, "unsupported well control");
+    }
+    SolveInfo si;
+    const int st = h->m->solve(T, mob, omega, gravity, bc, W, to_opts(o), p, flux, face_pressure, well_p,
+                               well_flux, si);
+    if (info) {
+      info->iterations = si.iterations;
+      info->restarts = si.restarts;
+      info->rel_residual = si.rel_residual;
+      info->setup_ms = si.setup_ms;
+      info->solve_ms = si.solve_ms;
+      info->levels = si.levels;
+      info->coarse_n = si.coarse_n;
+    }
+    if (st < 0) {
+      g_err = "pressure solve produced a non-finite residual";
+      return -2;
+    }
+    return st;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
 }
 
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z) {

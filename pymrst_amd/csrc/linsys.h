@@ -62,11 +62,11 @@ struct LinearSolverOptions {
   int maxiter = 200;
   int precision = 64;        // 64: FP64 preconditioner; 32: FP32 preconditioner
   int use_pressure_stage = 1;
-  int pre_smooth = 1, post_smooth = 1;
+  int pre_smooth = 2, post_smooth = 2;
   double jacobi_omega = 0.67;
-  double coarse_alpha = 1.0; // coarse-correction scaling
+  double coarse_alpha = 1.5; // coarse-correction scaling (over-correction for unsmoothed aggregation)
   int amg_cycles = 1;
-  double theta = 0.25;
+  double theta = 0.5;
   int coarse_size = 128;
   int pairwise_passes = 0;
   int reorthogonalize = 1;
@@ -81,6 +81,7 @@ struct SolveInfo {
   double setup_ms = 0.0, solve_ms = 0.0;
   int levels = 0;
   int coarse_n = 0;
+  int reorthogonalizations = 0;
 };
 
 class CprSolver;   // opaque (linsys.hip)

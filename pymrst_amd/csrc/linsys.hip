@@ -323,13 +323,18 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot(size_t n, int nv, cons
   }
 }
 
-// out[d] = sum_blk partial[blk*nd + d]   (one thread per d, fixed order)
-__global__ void k_reduce_partials(int nblk, int nd, const double* __restrict__ partial, double* __restrict__ out) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+// out[d] = sum_blk partial[blk*nd + d]: one 256-wide block per d, fixed
+// order within each thread's stride and a fixed shuffle tree, so the result
+// does not depend on timing.
+__global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int nd, const double* __restrict__ partial,
+                                                         double* __restrict__ out) {
+  __shared__ double sh[4];
+  const int d = blockIdx.x;
   if (d >= nd) return;
   double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * nd + d];
-  out[d] = s;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(size_t)b * nd + d];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[d] = s;
 }
 
 // w -= sum_d coef[d] V_d ; partial[blk] = sum of w^2 over the block's range.
@@ -348,6 +353,16 @@ __global__ __launch_bounds__(kRedThreads) void k_multiaxpy_norm(size_t n, int nv
   }
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// w = (w - sum_d coef[d] V_d) * scale
+__global__ void k_multiaxpy_scale(size_t n, int nv, const double* __restrict__ V, size_t ld,
+                                  const double* __restrict__ coef, double scale, double* __restrict__ w) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double wi = w[i];
+  for (int d = 0; d < nv; ++d) wi = fma(-coef[d], V[(size_t)d * ld + i], wi);
+  w[i] = wi * scale;
 }
 
 // x += sum_d coef[d] Z_d
@@ -379,7 +394,7 @@ double device_dot(const double* a, const double* b, size_t n, double* partial, h
   // partial needs kRedBlocks*2 + 2 doubles
   hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, 1, a, n, b, partial);
   launch_check();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, kRedBlocks, 2, partial,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(2), dim3(256), 0, s, kRedBlocks, 2, partial,
                      partial + (size_t)kRedBlocks * 2);
   launch_check();
   double h[2];
@@ -718,68 +733,116 @@ __global__ void k_cpr_mixfix_r2(int npc, const int* __restrict__ pc_cell, const 
   }
 }
 
-// z = P xp + (block-tridiagonal line solve of r2).
-template <int B, class T>
-__global__ void k_line_solve(int nl, const int* __restrict__ len, const int* __restrict__ cell,
-                             const T* __restrict__ inv, const T* __restrict__ Lo, const T* __restrict__ Up,
-                             const T* __restrict__ r2, const T* __restrict__ xp, double* __restrict__ z) {
+// z = P xp + (block-tridiagonal line solve of r2).  Each thread walks one
+// line; the loads of the next D positions are issued ahead of the dependent
+// block recurrence (a register ring, unrolled so it stays in registers).
+template <int B, class T, int D>
+__global__ __launch_bounds__(64) void k_line_solve(int nl, const int* __restrict__ len, const int* __restrict__ cell,
+                                                   const T* __restrict__ inv, const T* __restrict__ Lo,
+                                                   const T* __restrict__ Up, const T* __restrict__ r2,
+                                                   const T* __restrict__ xp, double* __restrict__ z) {
+  constexpr int BB = B * B;
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nl) return;
   const int L = len[l];
+  T ri[D][BB], rl[D][BB], rr[D][B];
+  int rc[D];
+  auto fload = [&](int k, T (&vi)[BB], T (&vl)[BB], T (&vr)[B], int& c) {
+    if (k >= L) return;
+    const size_t at = (size_t)k * nl + l;
+    c = cell[at];
+#pragma unroll
+    for (int q = 0; q < BB; ++q) {
+      vi[q] = inv[at * BB + q];
+      vl[q] = k > 0 ? Lo[at * BB + q] : (T)0;
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) vr[a] = r2[(size_t)c * B + a];
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) fload(u, ri[u], rl[u], rr[u], rc[u]);
   double y[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) y[a] = 0.0;
-  for (int k = 0; k < L; ++k) {
-    const size_t at = (size_t)k * nl + l;
-    const int c = cell[at];
-    double rhs[B];
+  for (int k0 = 0; k0 < L; k0 += D) {
 #pragma unroll
-    for (int a = 0; a < B; ++a) rhs[a] = (double)r2[(size_t)c * B + a];
-    if (k > 0) {
+    for (int u = 0; u < D; ++u) {
+      const int k = k0 + u;
+      if (k < L) {
+        double rhs[B];
 #pragma unroll
-      for (int a = 0; a < B; ++a)
+        for (int a = 0; a < B; ++a) {
+          double t = (double)rr[u][a];
 #pragma unroll
-        for (int q = 0; q < B; ++q) rhs[a] -= (double)Lo[at * B * B + a * B + q] * y[q];
+          for (int q = 0; q < B; ++q) t -= (double)rl[u][a * B + q] * y[q];
+          rhs[a] = t;
+        }
+        double yn[B];
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          double t = 0.0;
+#pragma unroll
+          for (int q = 0; q < B; ++q) t += (double)ri[u][a * B + q] * rhs[q];
+          yn[a] = t;
+        }
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          y[a] = yn[a];
+          z[(size_t)rc[u] * B + a] = yn[a];
+        }
+        fload(k + D, ri[u], rl[u], rr[u], rc[u]);
+      }
     }
-#pragma unroll
-    for (int a = 0; a < B; ++a) {
-      double s = 0.0;
-#pragma unroll
-      for (int q = 0; q < B; ++q) s += (double)inv[at * B * B + a * B + q] * rhs[q];
-      z[(size_t)c * B + a] = s;
-    }
-#pragma unroll
-    for (int a = 0; a < B; ++a) y[a] = z[(size_t)c * B + a];
   }
   // backward: x_k = y_k - D'_k^{-1} U_k x_{k+1};  z = x + P xp
-  double xn[B];
-  for (int k = L - 1; k >= 0; --k) {
+  T bi[D][BB], bu[D][BB];
+  double by[D][B], bx[D];
+  int bc[D];
+  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], double (&vy)[B], double& vx, int& c) {
+    if (k < 0) return;
     const size_t at = (size_t)k * nl + l;
-    const int c = cell[at];
-    double x[B];
+    c = cell[at];
 #pragma unroll
-    for (int a = 0; a < B; ++a) x[a] = z[(size_t)c * B + a];
-    if (k + 1 < L) {
-      double u[B];
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < B; ++q) s += (double)Up[at * B * B + a * B + q] * xn[q];
-        u[a] = s;
-      }
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < B; ++q) s += (double)inv[at * B * B + a * B + q] * u[q];
-        x[a] -= s;
-      }
+    for (int q = 0; q < BB; ++q) {
+      vi[q] = inv[at * BB + q];
+      vu[q] = k + 1 < L ? Up[at * BB + q] : (T)0;
     }
 #pragma unroll
-    for (int a = 0; a < B; ++a) {
-      xn[a] = x[a];
-      z[(size_t)c * B + a] = x[a] + (a == 0 ? (double)xp[c] : 0.0);
+    for (int a = 0; a < B; ++a) vy[a] = z[(size_t)c * B + a];
+    vx = (double)xp[c];
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) bload(L - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
+  double xn[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) xn[a] = 0.0;
+  for (int k0 = L - 1; k0 >= 0; k0 -= D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        double t[B], x[B];
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < B; ++q) s += (double)bu[u][a * B + q] * xn[q];
+          t[a] = s;
+        }
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < B; ++q) s += (double)bi[u][a * B + q] * t[q];
+          x[a] = by[u][a] - s;
+        }
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          xn[a] = x[a];
+          z[(size_t)bc[u] * B + a] = x[a] + (a == 0 ? bx[u] : 0.0);
+        }
+        bload(k - D, bi[u], bu[u], by[u], bx[u], bc[u]);
+      }
     }
   }
 }
@@ -1228,7 +1291,7 @@ static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* 
                          S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p);
       launch_check();
     }
-    hipLaunchKernelGGL((k_line_solve<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
+    hipLaunchKernelGGL((k_line_solve<B, T, 8>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
                        C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p,
                        V::x(f).p, z);
     launch_check();
@@ -1310,7 +1373,11 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   PMX_HIP(hipMemcpyAsync(C->r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   int total = 0;
   int status = 1;
-  while (total < o.maxiter) {
+  int cycles = 0;
+  // maxiter counts restart cycles, as scipy's gmres does (the reference's
+  // CPRSolver calls gmres(restart=50, maxiter=200))
+  while (cycles < o.maxiter) {
+    ++cycles;
     // V0 = r / beta
     hipLaunchKernelGGL(k_scale_copy, dim3(nb), dim3(256), 0, s, n, 1.0 / beta, C->r.p, C->V.p);
     launch_check();
@@ -1318,7 +1385,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     g[0] = beta;
     int j = 0;
     double resid = beta;
-    for (; j < m && total < o.maxiter; ++j) {
+    for (; j < m; ++j) {
       double* vj = C->V.p + (size_t)j * ld;
       double* zj = C->Z.p + (size_t)j * ld;
       double* w = C->V.p + (size_t)(j + 1) * ld;
@@ -1326,38 +1393,44 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       cpr_apply(C, S, vj, zj, o, s);
       C->prec_t.stop(s);
       system_matvec(S, zj, w, hscr, s, &C->spmv_t);
-      // CGS pass 1: h = V^T w
+      // Classical Gram-Schmidt: h = V^T w and w.w in one pass; the norm of
+      // the projected vector follows from Pythagoras, and a second pass runs
+      // only when it shows heavy cancellation ("twice is enough").
       const int nd = j + 2;
       hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w,
                          C->partial.p);
       launch_check();
-      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, kRedBlocks, nd, C->partial.p, C->hdev.p);
+      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, kRedBlocks, nd, C->partial.p, C->hdev.p);
       launch_check();
-      hipLaunchKernelGGL(k_multiaxpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld,
-                         C->hdev.p, w, C->partial.p);
-      launch_check();
-      if (o.reorthogonalize) {
-        // CGS pass 2 (coefficients added to the first pass's)
+      std::vector<double> hh(nd, 0.0);
+      PMX_HIP(hipMemcpyAsync(hh.data(), C->hdev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+      PMX_HIP(hipStreamSynchronize(s));
+      double hsq = 0.0;
+      for (int i = 0; i <= j; ++i) hsq += hh[i] * hh[i];
+      double wn2 = hh[j + 1] - hsq;
+      const double* coef = C->hdev.p;
+      for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
+      if (o.reorthogonalize && !(wn2 > 0.5 * hh[j + 1])) {
+        hipLaunchKernelGGL(k_multiaxpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld,
+                           C->hdev.p, w, C->partial.p);
+        launch_check();
         hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w,
                            C->partial.p);
         launch_check();
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, kRedBlocks, nd, C->partial.p,
-                           C->ydev.p);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, kRedBlocks, nd, C->partial.p, C->ydev.p);
         launch_check();
-        hipLaunchKernelGGL(k_multiaxpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld,
-                           C->ydev.p, w, C->partial.p);
-        launch_check();
+        std::vector<double> h2(nd, 0.0);
+        PMX_HIP(hipMemcpyAsync(h2.data(), C->ydev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+        PMX_HIP(hipStreamSynchronize(s));
+        double h2sq = 0.0;
+        for (int i = 0; i <= j; ++i) {
+          hcol[i] += h2[i];
+          h2sq += h2[i] * h2[i];
+        }
+        wn2 = h2[j + 1] - h2sq;
+        coef = C->ydev.p;
+        ++info.reorthogonalizations;
       }
-      hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, kRedBlocks, 1, C->partial.p, red);
-      launch_check();
-      std::vector<double> hh(nd + 1, 0.0), h2(nd, 0.0);
-      PMX_HIP(hipMemcpyAsync(hh.data(), C->hdev.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
-      if (o.reorthogonalize)
-        PMX_HIP(hipMemcpyAsync(h2.data(), C->ydev.p, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, s));
-      double wn2 = 0.0;
-      PMX_HIP(hipMemcpyAsync(&wn2, red, sizeof(double), hipMemcpyDeviceToHost, s));
-      PMX_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i <= j; ++i) hcol[i] = hh[i] + h2[i];
       hcol[j + 1] = std::sqrt(std::max(wn2, 0.0));
       ++total;
       if (!std::isfinite(hcol[j + 1])) { status = -1; break; }
@@ -1380,8 +1453,8 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       const double hnext = std::sqrt(std::max(wn2, 0.0));
       if (o.verbose > 1) fprintf(stderr, "[pmx] fgmres it %d  rel %.3e\n", total, resid / bnorm);
       if (resid <= tol || !(hnext > 0.0)) { ++j; break; }
-      // V_{j+1} = w / ||w||
-      hipLaunchKernelGGL(k_scale_inplace, dim3(nb), dim3(256), 0, s, n, 1.0 / hnext, w);
+      // V_{j+1} = (w - V h) / ||w - V h||
+      hipLaunchKernelGGL(k_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, j + 1, C->V.p, ld, coef, 1.0 / hnext, w);
       launch_check();
     }
     if (status == -1) break;
@@ -1408,6 +1481,9 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   info.levels = (int)C->L.size();
   info.coarse_n = C->coarse_n;
   info.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (o.verbose)
+    fprintf(stderr, "[pmx] fgmres (FP%d preconditioner): %d its (%d restarts), rel residual %.2e, %.2f ms, status %d\n",
+            C->prec_setup, total, info.restarts, info.rel_residual, info.solve_ms, status);
   return status;
 }
 

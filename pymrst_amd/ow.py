@@ -101,11 +101,11 @@ class CPRSolver:
     restart: int = 50
     precision: int = 64
     use_pressure_stage: bool = True
-    pre_smooth: int = 1
-    post_smooth: int = 1
+    pre_smooth: int = 2
+    post_smooth: int = 2
     jacobi_omega: float = 0.67
-    coarse_alpha: float = 1.0
-    theta: float = 0.25
+    coarse_alpha: float = 1.5
+    theta: float = 0.5
     coarse_size: int = 128
     pairwise_passes: int = 0
     reorthogonalize: bool = True

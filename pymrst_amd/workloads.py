@@ -6,8 +6,9 @@ layered lognormal permeability (per-layer median log-uniform on 1-1000 mD,
 in-layer sigma(ln k) = 1.5, kz/kx = 0.1), porosity 0.05 + 0.25 Phi(z)
 correlated with ln k, Corey 2/2 relative permeabilities, mu_w = 0.3 cP,
 mu_o = 3 cP, c_w = c_o = 1e-9 / Pa, c_r = 1e-10 / Pa, a five-spot (one
-centre rate injector, four corner bhp producers), gravity on, 10 steps of
-30 days.  Everything is drawn from one seeded generator, so the same inputs
+centre water injector at 5000 bbl/day -- the SPE10 Model 2 rate for this
+SPE10 Model 2 geometry -- and four corner producers at 150 bar), initial
+pressure 200 bar and water saturation 0.2, gravity on, 10 steps of 30 days.  Everything is drawn from one seeded generator, so the same inputs
 can be rebuilt anywhere.
 """
 from dataclasses import dataclass
@@ -41,8 +42,11 @@ class OWCase:
         return int(np.prod(self.dims))
 
 
+STB = 42.0 * 231.0 * 0.0254 ** 3    # stock-tank barrel in m^3
+
+
 def field_waterflood(dims=(60, 220, 85), cell=(6.1, 3.05, 0.61), seed=1, steps=10, step_days=30.0,
-                     inj_rate=0.025, prod_bhp=150 * BARSA) -> OWCase:
+                     inj_rate=5000.0 * STB / DAY, prod_bhp=150 * BARSA) -> OWCase:
     nx, ny, nz = dims
     nc = nx * ny * nz
     rng = np.random.default_rng(seed)
