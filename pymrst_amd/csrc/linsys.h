@@ -32,6 +32,7 @@ struct BlockSystem {
   // wells
   DArr<int> perf_cell, perf_well, well_ptr;  // perforations grouped by well
   DArr<int> pc_cell, pc_ptr, pc_perf;        // unique perforated cells -> perforations
+  DArr<int> pc_of;                           // cell -> index into pc_cell, or -1
   int npc = 0;
   DArr<double> Bp, Cp, Um, V, Dp;
   size_t n() const { return (size_t)nc * B + (size_t)nw * M; }

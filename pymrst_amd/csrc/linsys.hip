@@ -31,6 +31,7 @@ void DArr<T>::zero(hipStream_t s) {
 template struct DArr<double>;
 template struct DArr<float>;
 template struct DArr<int>;
+template struct DArr<unsigned char>;
 
 int grid_for(size_t threads, int block) { return (int)((threads + block - 1) / block); }
 
@@ -73,6 +74,13 @@ void BlockSystem::set_wells(int nw_, const std::vector<int>& perf_ptr, const std
   }
   if (!cells.empty()) ptr.push_back((int)perf.size());
   npc = (int)cells.size();
+  PMX_CHECK(nc > 0, "set the pattern before the wells");
+  std::vector<int> of(nc, -1);
+  for (int u = 0; u < npc; ++u) {
+    PMX_CHECK(cells[u] >= 0 && cells[u] < nc, "perforated cell outside the grid");
+    of[cells[u]] = u;
+  }
+  pc_of.upload(of, s);
   pc_cell.upload(cells, s);
   pc_ptr.upload(ptr, s);
   pc_perf.upload(perf, s);
@@ -409,7 +417,7 @@ double device_dot(const double* a, const double* b, size_t n, double* partial, h
 // Pressure matrix A_p = W J P and the pressure column of J (per block).
 template <int B, class T>
 __global__ void k_pressure_matrix(int nnzb, const double* __restrict__ val, const double* __restrict__ w,
-                                  T* __restrict__ Ap, T* __restrict__ Jp) {
+                                  T* __restrict__ Ap, T* __restrict__ Jp, T* __restrict__ Afull) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnzb) return;
   double s = 0.0;
@@ -420,6 +428,10 @@ __global__ void k_pressure_matrix(int nnzb, const double* __restrict__ val, cons
     Jp[(size_t)k * B + a] = (T)v;
   }
   Ap[k] = (T)s;
+  if (Afull != nullptr) {
+#pragma unroll
+    for (int q = 0; q < B * B; ++q) Afull[(size_t)k * B * B + q] = (T)val[(size_t)k * B * B + q];
+  }
 }
 
 // Small dense inverse (Gauss-Jordan with partial pivoting) in registers.
@@ -686,163 +698,192 @@ __global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __
   }
 }
 
-// r2 = r - J P xp on the cell rows (LPR lanes per row).
-template <int B, class T, int LPR>
-__global__ __launch_bounds__(256) void k_cpr_residual(int nc, const int* __restrict__ rowptr,
-                                                      const int* __restrict__ col, const T* __restrict__ Jp,
-                                                      const T* __restrict__ xp, const double* __restrict__ r,
-                                                      T* __restrict__ r2) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = t / LPR, lane = t % LPR;
+// Line Gauss-Seidel local stage, colour `color`: the right-hand side of every
+// cell on a line of this colour, written in line order (position-major):
+//   r2 = r - J P xp - sum_{j of an earlier colour} J_cj y_j - sum Um_j hp_w
+// where y holds the line solutions of the earlier colours and the last term
+// is the wells' mixture coupling of P xp on perforated cells.  LPR lanes per
+// row; xpl receives xp in the same order.
+template <int B, int X, class T, int LPR>
+__global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, int Lc, int color,
+                                                    const int* __restrict__ len, const int* __restrict__ cell,
+                                                    const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                    const T* __restrict__ Jp, const T* __restrict__ Af,
+                                                    const unsigned char* __restrict__ ccol,
+                                                    const T* __restrict__ xp, const T* __restrict__ ycell,
+                                                    const double* __restrict__ r, const int* __restrict__ pc_of,
+                                                    const int* __restrict__ pc_ptr, const int* __restrict__ pc_perf,
+                                                    const int* __restrict__ perf_well, const double* __restrict__ Um,
+                                                    const double* __restrict__ hp, T* __restrict__ r2l,
+                                                    T* __restrict__ xpl) {
+  constexpr int BB = B * B;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t slot = t / LPR;
+  const int lane = (int)(t % LPR);
+  const int li = (int)(slot % nlc), k = (int)(slot / nlc);
+  const int l = lc0 + li;
+  const bool valid = k < Lc && k < len[l];
+  const size_t at = (size_t)k * nl + l;
+  const int row = valid ? cell[at] : 0;
   double acc[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) acc[a] = 0.0;
-  if (row < nc) {
+  if (valid) {
     const int e = rowptr[row + 1];
-    for (int k = rowptr[row] + lane; k < e; k += LPR) {
-      const double xc = (double)xp[col[k]];
+    for (int q = rowptr[row] + lane; q < e; q += LPR) {
+      const int j = col[q];
+      const double xj = (double)xp[j];
+      if (color > 0 && ccol[j] < color) {
+        double v[B];
 #pragma unroll
-      for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)k * B + a], xc, acc[a]);
+        for (int b = 0; b < B; ++b) v[b] = (double)ycell[(size_t)j * B + b] + (b == 0 ? xj : 0.0);
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+          for (int b = 0; b < B; ++b) acc[a] = fma((double)Af[(size_t)q * BB + a * B + b], v[b], acc[a]);
+      } else {
+#pragma unroll
+        for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
+      }
     }
   }
 #pragma unroll
   for (int a = 0; a < B; ++a) acc[a] = wave_sum(acc[a], LPR);
-  if (row < nc && lane == 0) {
+  if (valid && lane == 0) {
+    double r2[B];
 #pragma unroll
-    for (int a = 0; a < B; ++a) r2[(size_t)row * B + a] = (T)(r[(size_t)row * B + a] - acc[a]);
-  }
-}
-
-// Mixture coupling of stage 2 on perforated cells: r2_c -= sum_j Um_j hp_wj.
-template <int B, int X, class T>
-__global__ void k_cpr_mixfix_r2(int npc, const int* __restrict__ pc_cell, const int* __restrict__ pc_ptr,
-                                const int* __restrict__ pc_perf, const int* __restrict__ perf_well,
-                                const double* __restrict__ Um, const double* __restrict__ hp, T* __restrict__ r2) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= npc) return;
-  const int c = pc_cell[u];
-  for (int t = pc_ptr[u]; t < pc_ptr[u + 1]; ++t) {
-    const int j = pc_perf[t], w = perf_well[j];
+    for (int a = 0; a < B; ++a) r2[a] = r[(size_t)row * B + a] - acc[a];
+    if (pc_of != nullptr) {
+      const int u = pc_of[row];
+      if (u >= 0)
+        for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
+          const int j = pc_perf[m], w = perf_well[j];
 #pragma unroll
-    for (int a = 0; a < B; ++a) {
-      double s = 0.0;
+          for (int a = 0; a < B; ++a)
 #pragma unroll
-      for (int x = 0; x < X; ++x) s += Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
-      r2[(size_t)c * B + a] = (T)((double)r2[(size_t)c * B + a] - s);
+            for (int x = 0; x < X; ++x) r2[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
+        }
     }
+#pragma unroll
+    for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)r2[a];
+    xpl[at] = xp[row];
   }
 }
 
-// z = P xp + (block-tridiagonal line solve of r2).  Each thread walks one
-// line; the loads of the next D positions are issued ahead of the dependent
-// block recurrence (a register ring, unrolled so it stays in registers).
+// Block-tridiagonal solves of the lines of one colour, one thread per line:
+// forward y_k = D'_k^{-1} (r2_k - L_k y_{k-1}) (y overwrites r2 in place),
+// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}; then z = P xp + x and, when
+// ycell is given, y_cell = x for the colours that follow.  A wave walks the
+// padded length of its longest line (lines are sorted by length, tables are
+// zero past each line's end and every colour fills whole waves), so every
+// load address is known without a data-dependent index and the next D
+// positions are in flight while the recurrence runs; the scheduling
+// barriers keep the compiler from sinking those loads next to their use.
+template <int B, class T>
+constexpr int line_depth() {
+  return B * B * (int)sizeof(T) <= 16 ? 16 : (B * B * (int)sizeof(T) <= 36 ? 8 : 4);
+}
+
 template <int B, class T, int D>
-__global__ __launch_bounds__(64) void k_line_solve(int nl, const int* __restrict__ len, const int* __restrict__ cell,
-                                                   const T* __restrict__ inv, const T* __restrict__ Lo,
-                                                   const T* __restrict__ Up, const T* __restrict__ r2,
-                                                   const T* __restrict__ xp, double* __restrict__ z) {
+__global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __restrict__ len,
+                                                const int* __restrict__ cell, const T* __restrict__ inv,
+                                                const T* __restrict__ Lo, const T* __restrict__ Up,
+                                                T* __restrict__ r2l, const T* __restrict__ xpl,
+                                                double* __restrict__ z, T* __restrict__ ycell) {
+  static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
   constexpr int BB = B * B;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nl) return;
-  const int L = len[l];
+  const int l = lc0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int Lp = (len[lc0 + blockIdx.x * blockDim.x] + D - 1) / D * D;
   T ri[D][BB], rl[D][BB], rr[D][B];
-  int rc[D];
-  auto fload = [&](int k, T (&vi)[BB], T (&vl)[BB], T (&vr)[B], int& c) {
-    if (k >= L) return;
+  auto fload = [&](int k, T (&vi)[BB], T (&vl)[BB], T (&vr)[B]) {
     const size_t at = (size_t)k * nl + l;
-    c = cell[at];
 #pragma unroll
     for (int q = 0; q < BB; ++q) {
       vi[q] = inv[at * BB + q];
-      vl[q] = k > 0 ? Lo[at * BB + q] : (T)0;
+      vl[q] = Lo[at * BB + q];
     }
 #pragma unroll
-    for (int a = 0; a < B; ++a) vr[a] = r2[(size_t)c * B + a];
+    for (int a = 0; a < B; ++a) vr[a] = r2l[at * B + a];
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) fload(u, ri[u], rl[u], rr[u], rc[u]);
+  for (int u = 0; u < D; ++u) fload(u, ri[u], rl[u], rr[u]);
   double y[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) y[a] = 0.0;
-  for (int k0 = 0; k0 < L; k0 += D) {
+  for (int k0 = 0; k0 < Lp; k0 += D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int k = k0 + u;
-      if (k < L) {
-        double rhs[B];
+      double rhs[B];
 #pragma unroll
-        for (int a = 0; a < B; ++a) {
-          double t = (double)rr[u][a];
+      for (int a = 0; a < B; ++a) {
+        double v = (double)rr[u][a];
 #pragma unroll
-          for (int q = 0; q < B; ++q) t -= (double)rl[u][a * B + q] * y[q];
-          rhs[a] = t;
-        }
-        double yn[B];
-#pragma unroll
-        for (int a = 0; a < B; ++a) {
-          double t = 0.0;
-#pragma unroll
-          for (int q = 0; q < B; ++q) t += (double)ri[u][a * B + q] * rhs[q];
-          yn[a] = t;
-        }
-#pragma unroll
-        for (int a = 0; a < B; ++a) {
-          y[a] = yn[a];
-          z[(size_t)rc[u] * B + a] = yn[a];
-        }
-        fload(k + D, ri[u], rl[u], rr[u], rc[u]);
+        for (int q = 0; q < B; ++q) v -= (double)rl[u][a * B + q] * y[q];
+        rhs[a] = v;
       }
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += (double)ri[u][a * B + q] * rhs[q];
+        y[a] = v;
+      }
+      const size_t at = (size_t)k * nl + l;
+#pragma unroll
+      for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)y[a];
+      fload(min(k + D, Lp - 1), ri[u], rl[u], rr[u]);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // backward: x_k = y_k - D'_k^{-1} U_k x_{k+1};  z = x + P xp
-  T bi[D][BB], bu[D][BB];
-  double by[D][B], bx[D];
+  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
   int bc[D];
-  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], double (&vy)[B], double& vx, int& c) {
-    if (k < 0) return;
+  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
     const size_t at = (size_t)k * nl + l;
-    c = cell[at];
 #pragma unroll
     for (int q = 0; q < BB; ++q) {
       vi[q] = inv[at * BB + q];
-      vu[q] = k + 1 < L ? Up[at * BB + q] : (T)0;
+      vu[q] = Up[at * BB + q];
     }
 #pragma unroll
-    for (int a = 0; a < B; ++a) vy[a] = z[(size_t)c * B + a];
-    vx = (double)xp[c];
+    for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
+    vx = xpl[at];
+    c = cell[at];
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) bload(L - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
+  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
   double xn[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) xn[a] = 0.0;
-  for (int k0 = L - 1; k0 >= 0; k0 -= D) {
+  for (int k0 = Lp - 1; k0 >= 0; k0 -= D) {
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int k = k0 - u;
-      if (k >= 0) {
-        double t[B], x[B];
+      double tq[B];
 #pragma unroll
-        for (int a = 0; a < B; ++a) {
-          double s = 0.0;
+      for (int a = 0; a < B; ++a) {
+        double v = 0.0;
 #pragma unroll
-          for (int q = 0; q < B; ++q) s += (double)bu[u][a * B + q] * xn[q];
-          t[a] = s;
-        }
-#pragma unroll
-        for (int a = 0; a < B; ++a) {
-          double s = 0.0;
-#pragma unroll
-          for (int q = 0; q < B; ++q) s += (double)bi[u][a * B + q] * t[q];
-          x[a] = by[u][a] - s;
-        }
-#pragma unroll
-        for (int a = 0; a < B; ++a) {
-          xn[a] = x[a];
-          z[(size_t)bc[u] * B + a] = x[a] + (a == 0 ? bx[u] : 0.0);
-        }
-        bload(k - D, bi[u], bu[u], by[u], bx[u], bc[u]);
+        for (int q = 0; q < B; ++q) v += (double)bu[u][a * B + q] * xn[q];
+        tq[a] = v;
       }
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = (double)by[u][a];
+#pragma unroll
+        for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
+        xn[a] = v;
+      }
+      const int c = bc[u];
+      if (c >= 0) {
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          z[(size_t)c * B + a] = xn[a] + (a == 0 ? (double)bx[u] : 0.0);
+          if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
+        }
+      }
+      bload(max(k - D, 0), bi[u], bu[u], by[u], bx[u], bc[u]);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
@@ -995,11 +1036,13 @@ class CprSolver {
   const double* wdev = nullptr;       // pressure weights of the current setup
   EventTimer spmv_t, prec_t;
   // line local stage: position-major line tables and block-tridiagonal factors
-  int nlines = 0, maxlen = 0, longest = 0;
+  int nlines = 0, maxlen = 0, longest = 0, ncolor = 0;
+  std::vector<int> color_ptr, color_len;             // line range and longest line of each colour
   std::vector<int> user_line_ptr, user_line_cells;   // lines given by the caller
   DArr<int> ln_len, ln_cell, ln_lo, ln_up;
-  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64;
-  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32;
+  DArr<unsigned char> ln_color;                      // colour of each cell's line
+  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64;
+  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, A32;
 };
 
 void cpr_profile(CprSolver* C, bool on) {
@@ -1073,11 +1116,17 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
     build_lines(fine, weights, HL);
   C->nlines = HL.nlines;
   C->maxlen = HL.maxlen;
-  C->longest = HL.nlines ? HL.len[0] : 0;
+  C->longest = HL.longest;
+  C->ncolor = HL.ncolor;
+  C->color_ptr = HL.color_ptr;
+  C->color_len.assign(HL.ncolor, 0);
+  for (int c = 0; c < HL.ncolor; ++c)
+    if (HL.color_ptr[c + 1] > HL.color_ptr[c]) C->color_len[c] = HL.len[HL.color_ptr[c]];
   C->ln_len.upload(HL.len, s);
   C->ln_cell.upload(HL.cell, s);
   C->ln_lo.upload(HL.lo, s);
   C->ln_up.upload(HL.up, s);
+  C->ln_color.upload(HL.cell_color, s);
   C->coarse_n = C->L.back()->n;
   PMX_CHECK(C->coarse_n <= 4096, "coarsest AMG level is too large for the dense solve");
   C->dense.alloc((size_t)C->coarse_n * C->coarse_n);
@@ -1087,7 +1136,7 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   if (o.verbose) {
     fprintf(stderr, "[pmx] AMG hierarchy:");
     for (auto& lv : C->L) fprintf(stderr, " %d(%d)", lv->n, lv->nnz);
-    fprintf(stderr, "\n[pmx] lines: %d, longest %d\n", C->nlines, C->longest);
+    fprintf(stderr, "\n[pmx] lines: %d in %d colours, longest %d\n", C->nlines, C->ncolor, C->longest);
   }
 }
 
@@ -1109,6 +1158,8 @@ struct Lv<double> {
   static DArr<double>& lf_lo(CprSolver& c) { return c.lf_lo64; }
   static DArr<double>& lf_up(CprSolver& c) { return c.lf_up64; }
   static DArr<double>& r2(CprSolver& c) { return c.r2_64; }
+  static DArr<double>& xpl(CprSolver& c) { return c.xpl64; }
+  static DArr<double>& yc(CprSolver& c) { return c.yc64; }
 };
 template <>
 struct Lv<float> {
@@ -1124,6 +1175,8 @@ struct Lv<float> {
   static DArr<float>& lf_lo(CprSolver& c) { return c.lf_lo32; }
   static DArr<float>& lf_up(CprSolver& c) { return c.lf_up32; }
   static DArr<float>& r2(CprSolver& c) { return c.r2_32; }
+  static DArr<float>& xpl(CprSolver& c) { return c.xpl32; }
+  static DArr<float>& yc(CprSolver& c) { return c.yc32; }
 };
 
 template <int B, int M, int X, class T>
@@ -1145,18 +1198,33 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
   DArr<T>& inv = V::inv(*C);
   Jp.alloc((size_t)S.nnzb * B);
   inv.alloc((size_t)S.nc * B * B);
+  T* Afull = nullptr;
+  if constexpr (std::is_same<T, float>::value) {
+    if (o.local_stage == 1 && C->nlines > 0) {
+      C->A32.alloc((size_t)S.nnzb * B * B);
+      Afull = C->A32.p;
+    }
+  }
   hipLaunchKernelGGL((k_pressure_matrix<B, T>), dim3(grid_for(S.nnzb)), dim3(256), 0, s, S.nnzb, S.val.p, wdev,
-                     V::v(f).p, Jp.p);
+                     V::v(f).p, Jp.p, Afull);
   launch_check();
   hipLaunchKernelGGL((k_block_inverse<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, S.diag.p, S.val.p,
                      inv.p);
   launch_check();
   if (o.local_stage == 1 && C->nlines > 0) {
-    const size_t m = (size_t)C->maxlen * C->nlines * B * B;
+    const size_t pos = (size_t)C->maxlen * C->nlines, m = pos * B * B;
     V::lf_inv(*C).alloc(m);
     V::lf_lo(*C).alloc(m);
     V::lf_up(*C).alloc(m);
-    V::r2(*C).alloc((size_t)S.nc * B);
+    V::r2(*C).alloc(pos * B);
+    V::xpl(*C).alloc(pos);
+    V::yc(*C).alloc((size_t)S.nc * B);
+    // entries past each line's end stay zero: the solves run over padded lengths
+    V::lf_inv(*C).zero(s);
+    V::lf_lo(*C).zero(s);
+    V::lf_up(*C).zero(s);
+    V::r2(*C).zero(s);
+    V::xpl(*C).zero(s);
     hipLaunchKernelGGL((k_line_factor<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
                        C->ln_len.p, C->ln_cell.p, C->ln_lo.p, C->ln_up.p, S.diag.p, S.val.p, V::lf_inv(*C).p,
                        V::lf_lo(*C).p, V::lf_up(*C).p);
@@ -1283,18 +1351,23 @@ static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* 
     launch_check();
   }
   if (o.local_stage == 1 && C->nlines > 0) {
-    hipLaunchKernelGGL((k_cpr_residual<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                       S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, r, V::r2(*C).p);
-    launch_check();
-    if (S.nw > 0 && S.has_mix && S.npc > 0) {
-      hipLaunchKernelGGL((k_cpr_mixfix_r2<B, X, T>), dim3(grid_for(S.npc)), dim3(256), 0, s, S.npc, S.pc_cell.p,
-                         S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p);
+    const T* Af;
+    if constexpr (std::is_same<T, double>::value) Af = S.val.p; else Af = C->A32.p;
+    const bool mix = S.nw > 0 && S.has_mix && S.npc > 0;
+    for (int c = 0; c < C->ncolor; ++c) {
+      const int lc0 = C->color_ptr[c], nlc = C->color_ptr[c + 1] - lc0, Lc = C->color_len[c];
+      if (nlc == 0) continue;
+      hipLaunchKernelGGL((k_line_resid<B, X, T, LPR>), dim3(grid_for((size_t)nlc * Lc * LPR)), dim3(256), 0, s,
+                         C->nlines, lc0, nlc, Lc, c, C->ln_len.p, C->ln_cell.p, S.rowptr.p, S.col.p, V::Jp(*C).p,
+                         Af, C->ln_color.p, V::x(f).p, V::yc(*C).p, r, mix ? S.pc_of.p : nullptr, S.pc_ptr.p,
+                         S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p, V::xpl(*C).p);
+      launch_check();
+      PMX_CHECK(nlc % LINE_GROUP == 0, "line colours must fill whole waves");
+      hipLaunchKernelGGL((k_line_gs<B, T, line_depth<B, T>()>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s,
+                         C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p,
+                         V::xpl(*C).p, z, c + 1 < C->ncolor ? V::yc(*C).p : nullptr);
       launch_check();
     }
-    hipLaunchKernelGGL((k_line_solve<B, T, 8>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
-                       C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p,
-                       V::x(f).p, z);
-    launch_check();
     return;
   }
   hipLaunchKernelGGL((k_cpr_stage2<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,

@@ -91,10 +91,17 @@ void build_aggregation(const HostPattern& fine, const std::vector<double>& w,
 
 // Lines of strongly coupled cells, stored position-major (entry k*nlines + l
 // is the k-th cell of line l, -1 past its end); lo/up are the BCSR entries of
-// the blocks (c_k, c_{k-1}) and (c_k, c_{k+1}).
+// the blocks (c_k, c_{k-1}) and (c_k, c_{k+1}).  Lines are coloured so that
+// no two lines of one colour share a matrix entry; colour c owns lines
+// [color_ptr[c], color_ptr[c+1]), each range sorted by descending length and
+// padded with empty lines to a multiple of LINE_GROUP (one wave per group).
+// maxlen is padded to a multiple of LINE_PAD (tables hold maxlen rows).
+constexpr int LINE_PAD = 16;
+constexpr int LINE_GROUP = 64;
 struct HostLines {
-  int nlines = 0, maxlen = 0;
-  std::vector<int> len, cell, lo, up;
+  int nlines = 0, maxlen = 0, longest = 0, ncolor = 0;
+  std::vector<int> len, cell, lo, up, color_ptr;
+  std::vector<unsigned char> cell_color;   // colour of each cell's line
 };
 void build_lines(const HostPattern& P, const std::vector<double>& w, HostLines& L);
 void lines_from_list(const HostPattern& P, int nlines, const int* ptr, const int* cells, HostLines& L);

@@ -272,6 +272,83 @@ void build_aggregation(const HostPattern& fine, const std::vector<double>& w, co
 
 namespace pmx {
 
+// Position-major tables for a set of lines that covers every cell once.
+// Lines are coloured greedily (longest first) so that lines sharing a matrix
+// entry get different colours; a Gauss-Seidel sweep over the colours then
+// solves all lines of one colour at once.
+static void line_tables(const HostPattern& P, std::vector<std::vector<int>>& lines, HostLines& L) {
+  const int n = P.n;
+  std::stable_sort(lines.begin(), lines.end(),
+                   [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
+  int nl = (int)lines.size();
+  std::vector<int> line_of(n, -1);
+  for (int l = 0; l < nl; ++l)
+    for (int c : lines[l]) {
+      PMX_CHECK(line_of[c] < 0, "a cell lies on two lines");
+      line_of[c] = l;
+    }
+  for (int c = 0; c < n; ++c) PMX_CHECK(line_of[c] >= 0, "lines must cover every cell");
+  std::vector<int> color(nl, -1), mark;
+  int ncolor = 0;
+  for (int l = 0; l < nl; ++l) {
+    mark.assign(ncolor + 1, 0);
+    for (int c : lines[l])
+      for (int k = P.rowptr[c]; k < P.rowptr[c + 1]; ++k) {
+        const int m = line_of[P.col[k]];
+        if (m != l && color[m] >= 0) mark[color[m]] = 1;
+      }
+    int pick = 0;
+    while (mark[pick]) ++pick;
+    color[l] = pick;
+    ncolor = std::max(ncolor, pick + 1);
+  }
+  PMX_CHECK(ncolor <= 255, "too many line colours");
+  // colour-major order; within a colour the length order above is kept
+  std::vector<int> order(nl);
+  for (int l = 0; l < nl; ++l) order[l] = l;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return color[a] < color[b]; });
+  std::vector<std::vector<int>> sorted;
+  L.color_ptr.assign(1, 0);
+  L.cell_color.assign(n, 0);
+  for (int t = 0; t < nl;) {
+    const int c = color[order[t]];
+    for (; t < nl && color[order[t]] == c; ++t) {
+      for (int cell : lines[order[t]]) L.cell_color[cell] = (unsigned char)c;
+      sorted.push_back(std::move(lines[order[t]]));
+    }
+    while (sorted.size() % LINE_GROUP) sorted.emplace_back();   // empty lines fill the last wave
+    L.color_ptr.push_back((int)sorted.size());
+  }
+  nl = (int)sorted.size();
+  int longest = 0;
+  for (auto& ln : sorted) longest = std::max(longest, (int)ln.size());
+  const int maxlen = (longest + LINE_PAD - 1) / LINE_PAD * LINE_PAD;
+  L.nlines = nl;
+  L.ncolor = ncolor;
+  L.longest = longest;
+  L.maxlen = maxlen;
+  L.len.resize(nl);
+  L.cell.assign((size_t)maxlen * nl, -1);
+  L.lo.assign((size_t)maxlen * nl, -1);
+  L.up.assign((size_t)maxlen * nl, -1);
+  auto entry = [&](int r, int c) {
+    auto b0 = P.col.begin() + P.rowptr[r], b1 = P.col.begin() + P.rowptr[r + 1];
+    auto it = std::lower_bound(b0, b1, c);
+    PMX_CHECK(it != b1 && *it == c, "consecutive line cells are not connected");
+    return (int)(it - P.col.begin());
+  };
+  for (int l = 0; l < nl; ++l) {
+    const auto& ln = sorted[l];
+    L.len[l] = (int)ln.size();
+    for (size_t k = 0; k < ln.size(); ++k) {
+      const size_t at = k * nl + l;
+      L.cell[at] = ln[k];
+      if (k > 0) L.lo[at] = entry(ln[k], ln[k - 1]);
+      if (k + 1 < ln.size()) L.up[at] = entry(ln[k], ln[k + 1]);
+    }
+  }
+}
+
 // Lines of strongly coupled cells: each cell links to its (at most) two
 // strongest neighbours; a link is kept when it is mutual, which leaves
 // paths and cycles; cycles are cut once.  On layered Cartesian grids with
@@ -318,43 +395,14 @@ void build_lines(const HostPattern& P, const std::vector<double>& w, HostLines& 
     if (!seen[i] && (nb0[i] < 0 || nb1[i] < 0)) lines.push_back(walk(i));
   for (int i = 0; i < n; ++i)   // what is left lies on cycles: cut at i
     if (!seen[i]) lines.push_back(walk(i));
-  // ascending first cell, then stable by descending length
   std::stable_sort(lines.begin(), lines.end(),
                    [](const std::vector<int>& a, const std::vector<int>& b) { return a[0] < b[0]; });
-  std::stable_sort(lines.begin(), lines.end(),
-                   [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
-  const int nl = (int)lines.size();
-  const int maxlen = nl ? (int)lines[0].size() : 0;
-  L.nlines = nl;
-  L.maxlen = maxlen;
-  L.len.resize(nl);
-  L.cell.assign((size_t)maxlen * nl, -1);
-  L.lo.assign((size_t)maxlen * nl, -1);
-  L.up.assign((size_t)maxlen * nl, -1);
-  auto entry = [&](int r, int c) {
-    auto b0 = P.col.begin() + P.rowptr[r], b1 = P.col.begin() + P.rowptr[r + 1];
-    auto it = std::lower_bound(b0, b1, c);
-    PMX_CHECK(it != b1 && *it == c, "line link outside the pattern");
-    return (int)(it - P.col.begin());
-  };
-  for (int l = 0; l < nl; ++l) {
-    const auto& ln = lines[l];
-    L.len[l] = (int)ln.size();
-    for (size_t k = 0; k < ln.size(); ++k) {
-      const size_t at = k * nl + l;
-      L.cell[at] = ln[k];
-      if (k > 0) L.lo[at] = entry(ln[k], ln[k - 1]);
-      if (k + 1 < ln.size()) L.up[at] = entry(ln[k], ln[k + 1]);
-    }
-  }
+  line_tables(P, lines, L);
 }
 
-}  // namespace pmx
-
-namespace pmx {
-
 // Lines given by the caller (line_ptr[nlines+1], cells[]), e.g. the columns
-// of a Cartesian grid; consecutive cells of a line must be connected.
+// of a Cartesian grid; consecutive cells of a line must be connected.  Cells
+// on no line become lines of one cell.
 void lines_from_list(const HostPattern& P, int nlines, const int* ptr, const int* cells, HostLines& L) {
   std::vector<std::vector<int>> lines(nlines);
   std::vector<char> seen(P.n, 0);
@@ -369,32 +417,7 @@ void lines_from_list(const HostPattern& P, int nlines, const int* ptr, const int
   }
   for (int i = 0; i < P.n; ++i)
     if (!seen[i]) lines.push_back(std::vector<int>{i});
-  std::stable_sort(lines.begin(), lines.end(),
-                   [](const std::vector<int>& a, const std::vector<int>& b) { return a.size() > b.size(); });
-  const int nl = (int)lines.size();
-  const int maxlen = nl ? (int)lines[0].size() : 0;
-  L.nlines = nl;
-  L.maxlen = maxlen;
-  L.len.resize(nl);
-  L.cell.assign((size_t)maxlen * nl, -1);
-  L.lo.assign((size_t)maxlen * nl, -1);
-  L.up.assign((size_t)maxlen * nl, -1);
-  auto entry = [&](int r, int c) {
-    auto b0 = P.col.begin() + P.rowptr[r], b1 = P.col.begin() + P.rowptr[r + 1];
-    auto it = std::lower_bound(b0, b1, c);
-    PMX_CHECK(it != b1 && *it == c, "consecutive line cells are not connected");
-    return (int)(it - P.col.begin());
-  };
-  for (int l = 0; l < nl; ++l) {
-    const auto& ln = lines[l];
-    L.len[l] = (int)ln.size();
-    for (size_t k = 0; k < ln.size(); ++k) {
-      const size_t at = k * nl + l;
-      L.cell[at] = ln[k];
-      if (k > 0) L.lo[at] = entry(ln[k], ln[k - 1]);
-      if (k + 1 < ln.size()) L.up[at] = entry(ln[k], ln[k + 1]);
-    }
-  }
+  line_tables(P, lines, L);
 }
 
 }  // namespace pmx
