@@ -39,7 +39,9 @@ typedef struct pmx_solver_options {
   int32_t pairwise_passes;
   int32_t reorthogonalize;
   int32_t verbose;
-  int32_t local_stage;      /* 0: cell block-Jacobi, 1: line block-tridiagonal solves */
+  int32_t local_stage;      /* 0: cell block-Jacobi, 1: coloured line Gauss-Seidel */
+  int32_t pressure_cycles;  /* V-cycles per pressure-stage application (stationary iteration) */
+  int32_t local_sweeps;     /* Gauss-Seidel sweeps over the line colours */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {

@@ -30,6 +30,8 @@ class SolverOptions(ctypes.Structure):
         ("reorthogonalize", ctypes.c_int32),
         ("verbose", ctypes.c_int32),
         ("local_stage", ctypes.c_int32),
+        ("pressure_cycles", ctypes.c_int32),
+        ("local_sweeps", ctypes.c_int32),
     ]
 
 

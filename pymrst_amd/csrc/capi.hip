@@ -53,6 +53,8 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.reorthogonalize = o->reorthogonalize;
   r.verbose = o->verbose;
   r.local_stage = o->local_stage;
+  r.amg_cycles = o->pressure_cycles;
+  r.local_sweeps = o->local_sweeps;
   return r;
 }
 
@@ -79,6 +81,8 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->reorthogonalize = d.reorthogonalize;
   o->verbose = d.verbose;
   o->local_stage = d.local_stage;
+  o->pressure_cycles = d.amg_cycles;
+  o->local_sweeps = d.local_sweeps;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {

@@ -72,7 +72,8 @@ struct LinearSolverOptions {
   int pairwise_passes = 0;
   int reorthogonalize = 1;
   int verbose = 0;
-  int local_stage = 1;       // 0: cell block-Jacobi; 1: line block-tridiagonal solves
+  int local_stage = 1;       // 0: cell block-Jacobi; 1: coloured line Gauss-Seidel
+  int local_sweeps = 4;      // Gauss-Seidel sweeps of the line local stage
 };
 
 struct SolveInfo {

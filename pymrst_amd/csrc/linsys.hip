@@ -698,12 +698,17 @@ __global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __
   }
 }
 
-// Line Gauss-Seidel local stage, colour `color`: the right-hand side of every
-// cell on a line of this colour, written in line order (position-major):
-//   r2 = r - J P xp - sum_{j of an earlier colour} J_cj y_j - sum Um_j hp_w
-// where y holds the line solutions of the earlier colours and the last term
-// is the wells' mixture coupling of P xp on perforated cells.  LPR lanes per
-// row; xpl receives xp in the same order.
+// Line Gauss-Seidel local stage.  The local system is A y = r1 on the cell
+// rows, r1 = r - J P xp - sum Um_j hp_w (the pressure correction and the
+// wells' mixture coupling of P xp removed); lines of one colour are solved
+// together and the colours are swept in order, several times if asked.
+//
+// First sweep, colour `color`: the right-hand side of every cell on a line
+// of this colour, written in line order (position-major):
+//   r2 = r1 - sum_{j of an earlier colour} J_cj y_j
+// where y holds the line solutions of the earlier colours (ycell, by cell).
+// r1 itself is kept in r1l when later sweeps need it; xpl receives xp in the
+// same order.  LPR lanes per row.
 template <int B, int X, class T, int LPR>
 __global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, int Lc, int color,
                                                     const int* __restrict__ len, const int* __restrict__ cell,
@@ -715,7 +720,73 @@ __global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, in
                                                     const int* __restrict__ pc_ptr, const int* __restrict__ pc_perf,
                                                     const int* __restrict__ perf_well, const double* __restrict__ Um,
                                                     const double* __restrict__ hp, T* __restrict__ r2l,
-                                                    T* __restrict__ xpl) {
+                                                    T* __restrict__ r1l, T* __restrict__ xpl) {
+  constexpr int BB = B * B;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t slot = t / LPR;
+  const int lane = (int)(t % LPR);
+  const int li = (int)(slot % nlc), k = (int)(slot / nlc);
+  const int l = lc0 + li;
+  const bool valid = k < Lc && k < len[l];
+  const size_t at = (size_t)k * nl + l;
+  const int row = valid ? cell[at] : 0;
+  double acc[B], accy[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) acc[a] = accy[a] = 0.0;
+  if (valid) {
+    const int e = rowptr[row + 1];
+    for (int q = rowptr[row] + lane; q < e; q += LPR) {
+      const int j = col[q];
+      const double xj = (double)xp[j];
+#pragma unroll
+      for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
+      if (color > 0 && ccol[j] < color) {
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+          for (int b = 0; b < B; ++b)
+            accy[a] = fma((double)Af[(size_t)q * BB + a * B + b], (double)ycell[(size_t)j * B + b], accy[a]);
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < B; ++a) {
+    acc[a] = wave_sum(acc[a], LPR);
+    accy[a] = wave_sum(accy[a], LPR);
+  }
+  if (valid && lane == 0) {
+    double r1[B];
+#pragma unroll
+    for (int a = 0; a < B; ++a) r1[a] = r[(size_t)row * B + a] - acc[a];
+    if (pc_of != nullptr) {
+      const int u = pc_of[row];
+      if (u >= 0)
+        for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
+          const int j = pc_perf[m], w = perf_well[j];
+#pragma unroll
+          for (int a = 0; a < B; ++a)
+#pragma unroll
+            for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)(r1[a] - accy[a]);
+    if (r1l != nullptr) {
+#pragma unroll
+      for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
+    }
+    xpl[at] = xp[row];
+  }
+}
+
+// Later sweeps, colour `color`: r2 = r1 - A y over the whole row, with y the
+// current local solution of every cell (ycell).
+template <int B, class T, int LPR>
+__global__ __launch_bounds__(256) void k_line_resid_sweep(int nl, int lc0, int nlc, int Lc,
+                                                          const int* __restrict__ len, const int* __restrict__ cell,
+                                                          const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                          const T* __restrict__ Af, const T* __restrict__ ycell,
+                                                          const T* __restrict__ r1l, T* __restrict__ r2l) {
   constexpr int BB = B * B;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t slot = t / LPR;
@@ -732,64 +803,46 @@ __global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, in
     const int e = rowptr[row + 1];
     for (int q = rowptr[row] + lane; q < e; q += LPR) {
       const int j = col[q];
-      const double xj = (double)xp[j];
-      if (color > 0 && ccol[j] < color) {
-        double v[B];
+      double yj[B];
 #pragma unroll
-        for (int b = 0; b < B; ++b) v[b] = (double)ycell[(size_t)j * B + b] + (b == 0 ? xj : 0.0);
+      for (int b = 0; b < B; ++b) yj[b] = (double)ycell[(size_t)j * B + b];
 #pragma unroll
-        for (int a = 0; a < B; ++a)
+      for (int a = 0; a < B; ++a)
 #pragma unroll
-          for (int b = 0; b < B; ++b) acc[a] = fma((double)Af[(size_t)q * BB + a * B + b], v[b], acc[a]);
-      } else {
-#pragma unroll
-        for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
-      }
+        for (int b = 0; b < B; ++b) acc[a] = fma((double)Af[(size_t)q * BB + a * B + b], yj[b], acc[a]);
     }
   }
 #pragma unroll
   for (int a = 0; a < B; ++a) acc[a] = wave_sum(acc[a], LPR);
   if (valid && lane == 0) {
-    double r2[B];
 #pragma unroll
-    for (int a = 0; a < B; ++a) r2[a] = r[(size_t)row * B + a] - acc[a];
-    if (pc_of != nullptr) {
-      const int u = pc_of[row];
-      if (u >= 0)
-        for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
-          const int j = pc_perf[m], w = perf_well[j];
-#pragma unroll
-          for (int a = 0; a < B; ++a)
-#pragma unroll
-            for (int x = 0; x < X; ++x) r2[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
-        }
-    }
-#pragma unroll
-    for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)r2[a];
-    xpl[at] = xp[row];
+    for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)((double)r1l[at * B + a] - acc[a]);
   }
 }
 
 // Block-tridiagonal solves of the lines of one colour, one thread per line:
 // forward y_k = D'_k^{-1} (r2_k - L_k y_{k-1}) (y overwrites r2 in place),
-// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}; then z = P xp + x and, when
-// ycell is given, y_cell = x for the colours that follow.  A wave walks the
-// padded length of its longest line (lines are sorted by length, tables are
-// zero past each line's end and every colour fills whole waves), so every
-// load address is known without a data-dependent index and the next D
-// positions are in flight while the recurrence runs; the scheduling
-// barriers keep the compiler from sinking those loads next to their use.
+// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}.  The first sweep's solution
+// is x; a later sweep's x is a correction added to the line's current
+// solution yl.  The new solution goes to yl and ycell when given, and
+// z = P xp + solution.  A wave walks the padded length of its longest line
+// (lines are sorted by length, tables are zero past each line's end and
+// every colour fills whole waves), so every load address is known without a
+// data-dependent index and the next D positions are in flight while the
+// recurrence runs; the scheduling barriers keep the compiler from sinking
+// those loads next to their use.
 template <int B, class T>
 constexpr int line_depth() {
   return B * B * (int)sizeof(T) <= 16 ? 16 : (B * B * (int)sizeof(T) <= 36 ? 8 : 4);
 }
 
-template <int B, class T, int D>
+template <int B, class T, int D, bool ACC>
 __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __restrict__ len,
                                                 const int* __restrict__ cell, const T* __restrict__ inv,
                                                 const T* __restrict__ Lo, const T* __restrict__ Up,
                                                 T* __restrict__ r2l, const T* __restrict__ xpl,
-                                                double* __restrict__ z, T* __restrict__ ycell) {
+                                                double* __restrict__ z, T* __restrict__ ycell,
+                                                T* __restrict__ yl) {
   static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
   constexpr int BB = B * B;
   const int l = lc0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -836,9 +889,9 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
+  T bi[D][BB], bu[D][BB], by[D][B], bo[D][B], bx[D];
   int bc[D];
-  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
+  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T (&vo)[B], T& vx, int& c) {
     const size_t at = (size_t)k * nl + l;
 #pragma unroll
     for (int q = 0; q < BB; ++q) {
@@ -846,12 +899,15 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       vu[q] = Up[at * BB + q];
     }
 #pragma unroll
-    for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
+    for (int a = 0; a < B; ++a) {
+      vy[a] = r2l[at * B + a];
+      if constexpr (ACC) vo[a] = yl[at * B + a];
+    }
     vx = xpl[at];
     c = cell[at];
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
+  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bo[u], bx[u], bc[u]);
   double xn[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) xn[a] = 0.0;
@@ -874,15 +930,23 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
         for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
         xn[a] = v;
       }
+      double sol[B];
+#pragma unroll
+      for (int a = 0; a < B; ++a) sol[a] = ACC ? (double)bo[u][a] + xn[a] : xn[a];
+      if (yl != nullptr) {
+        const size_t at = (size_t)k * nl + l;
+#pragma unroll
+        for (int a = 0; a < B; ++a) yl[at * B + a] = (T)sol[a];
+      }
       const int c = bc[u];
       if (c >= 0) {
 #pragma unroll
         for (int a = 0; a < B; ++a) {
-          z[(size_t)c * B + a] = xn[a] + (a == 0 ? (double)bx[u] : 0.0);
-          if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
+          z[(size_t)c * B + a] = sol[a] + (a == 0 ? (double)bx[u] : 0.0);
+          if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)sol[a];
         }
       }
-      bload(max(k - D, 0), bi[u], bu[u], by[u], bx[u], bc[u]);
+      bload(max(k - D, 0), bi[u], bu[u], by[u], bo[u], bx[u], bc[u]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -935,6 +999,12 @@ __global__ __launch_bounds__(256) void k_csr_sweep(int n, const int* __restrict_
     const T res = b[row] - acc;
     out[row] = mode == 0 ? x[row] + dinv[row] * res : res;
   }
+}
+
+template <class T>
+__global__ void k_add_inplace(int n, const T* __restrict__ a, T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += a[i];
 }
 
 template <class T>
@@ -1041,8 +1111,11 @@ class CprSolver {
   std::vector<int> user_line_ptr, user_line_cells;   // lines given by the caller
   DArr<int> ln_len, ln_cell, ln_lo, ln_up;
   DArr<unsigned char> ln_color;                      // colour of each cell's line
-  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64;
-  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, A32;
+  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64, yl64;
+  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, yl32, A32;
+  // repeated pressure cycles: original right-hand side and accumulated solution
+  DArr<double> pb64, pacc64;
+  DArr<float> pb32, pacc32;
 };
 
 void cpr_profile(CprSolver* C, bool on) {
@@ -1160,6 +1233,10 @@ struct Lv<double> {
   static DArr<double>& r2(CprSolver& c) { return c.r2_64; }
   static DArr<double>& xpl(CprSolver& c) { return c.xpl64; }
   static DArr<double>& yc(CprSolver& c) { return c.yc64; }
+  static DArr<double>& r1l(CprSolver& c) { return c.r1l64; }
+  static DArr<double>& yl(CprSolver& c) { return c.yl64; }
+  static DArr<double>& pb(CprSolver& c) { return c.pb64; }
+  static DArr<double>& pacc(CprSolver& c) { return c.pacc64; }
 };
 template <>
 struct Lv<float> {
@@ -1177,6 +1254,10 @@ struct Lv<float> {
   static DArr<float>& r2(CprSolver& c) { return c.r2_32; }
   static DArr<float>& xpl(CprSolver& c) { return c.xpl32; }
   static DArr<float>& yc(CprSolver& c) { return c.yc32; }
+  static DArr<float>& r1l(CprSolver& c) { return c.r1l32; }
+  static DArr<float>& yl(CprSolver& c) { return c.yl32; }
+  static DArr<float>& pb(CprSolver& c) { return c.pb32; }
+  static DArr<float>& pacc(CprSolver& c) { return c.pacc32; }
 };
 
 template <int B, int M, int X, class T>
@@ -1219,6 +1300,12 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::r2(*C).alloc(pos * B);
     V::xpl(*C).alloc(pos);
     V::yc(*C).alloc((size_t)S.nc * B);
+    if (o.local_sweeps > 1) {
+      V::r1l(*C).alloc(pos * B);
+      V::yl(*C).alloc(pos * B);
+      V::r1l(*C).zero(s);
+      V::yl(*C).zero(s);
+    }
     // entries past each line's end stay zero: the solves run over padded lengths
     V::lf_inv(*C).zero(s);
     V::lf_lo(*C).zero(s);
@@ -1328,9 +1415,22 @@ static void apply_t(CprSolver* C, const BlockSystem& S, const double* r, double*
   hipLaunchKernelGGL((k_weight_restrict<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, wdev, r, V::b(f).p);
   launch_check();
   vcycle<T>(C, 0, o, s);
-  for (int cyc = 1; cyc < o.amg_cycles; ++cyc) {
-    // further cycles on the pressure residual: b <- b - A x, correction added
-    throw std::invalid_argument("amg_cycles > 1 is not supported");
+  if (o.amg_cycles > 1) {
+    // stationary iteration with the V-cycle: x <- x + V(b - A_p x)
+    DArr<T>& b0 = V::pb(*C);
+    DArr<T>& acc = V::pacc(*C);
+    b0.alloc(f.n);
+    acc.alloc(f.n);
+    PMX_HIP(hipMemcpyAsync(b0.p, V::b(f).p, sizeof(T) * f.n, hipMemcpyDeviceToDevice, s));
+    for (int cyc = 1; cyc < o.amg_cycles; ++cyc) {
+      std::swap(V::x(f).p, acc.p);
+      hipLaunchKernelGGL((k_csr_sweep<T, LPR>), dim3(grid_for((size_t)f.n * LPR)), dim3(256), 0, s, f.n,
+                         f.rowptr.p, f.col.p, V::v(f).p, V::d(f).p, b0.p, acc.p, V::b(f).p, 1);
+      launch_check();
+      vcycle<T>(C, 0, o, s);
+      hipLaunchKernelGGL((k_add_inplace<T>), dim3(grid_for(f.n)), dim3(256), 0, s, f.n, acc.p, V::x(f).p);
+      launch_check();
+    }
   }
   stage2<B, M, X, T>(C, S, r, z, o, s);
 }
@@ -1354,19 +1454,37 @@ static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* 
     const T* Af;
     if constexpr (std::is_same<T, double>::value) Af = S.val.p; else Af = C->A32.p;
     const bool mix = S.nw > 0 && S.has_mix && S.npc > 0;
-    for (int c = 0; c < C->ncolor; ++c) {
-      const int lc0 = C->color_ptr[c], nlc = C->color_ptr[c + 1] - lc0, Lc = C->color_len[c];
-      if (nlc == 0) continue;
-      hipLaunchKernelGGL((k_line_resid<B, X, T, LPR>), dim3(grid_for((size_t)nlc * Lc * LPR)), dim3(256), 0, s,
-                         C->nlines, lc0, nlc, Lc, c, C->ln_len.p, C->ln_cell.p, S.rowptr.p, S.col.p, V::Jp(*C).p,
-                         Af, C->ln_color.p, V::x(f).p, V::yc(*C).p, r, mix ? S.pc_of.p : nullptr, S.pc_ptr.p,
-                         S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p, V::xpl(*C).p);
-      launch_check();
-      PMX_CHECK(nlc % LINE_GROUP == 0, "line colours must fill whole waves");
-      hipLaunchKernelGGL((k_line_gs<B, T, line_depth<B, T>()>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s,
-                         C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p,
-                         V::xpl(*C).p, z, c + 1 < C->ncolor ? V::yc(*C).p : nullptr);
-      launch_check();
+    const int sweeps = std::max(1, o.local_sweeps);
+    constexpr int D = line_depth<B, T>();
+    T* r1l = sweeps > 1 ? V::r1l(*C).p : nullptr;
+    T* yl = sweeps > 1 ? V::yl(*C).p : nullptr;
+    for (int sw = 0; sw < sweeps; ++sw) {
+      for (int c = 0; c < C->ncolor; ++c) {
+        const int lc0 = C->color_ptr[c], nlc = C->color_ptr[c + 1] - lc0, Lc = C->color_len[c];
+        if (nlc == 0) continue;
+        PMX_CHECK(nlc % LINE_GROUP == 0, "line colours must fill whole waves");
+        const int gr = grid_for((size_t)nlc * Lc * LPR);
+        T* yc = (sweeps > 1 || c + 1 < C->ncolor) ? V::yc(*C).p : nullptr;
+        if (sw == 0) {
+          hipLaunchKernelGGL((k_line_resid<B, X, T, LPR>), dim3(gr), dim3(256), 0, s, C->nlines, lc0, nlc, Lc, c,
+                             C->ln_len.p, C->ln_cell.p, S.rowptr.p, S.col.p, V::Jp(*C).p, Af, C->ln_color.p,
+                             V::x(f).p, V::yc(*C).p, r, mix ? S.pc_of.p : nullptr, S.pc_ptr.p, S.pc_perf.p,
+                             S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p, r1l, V::xpl(*C).p);
+          launch_check();
+          hipLaunchKernelGGL((k_line_gs<B, T, D, false>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s,
+                             C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
+                             V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, z, yc, yl);
+          launch_check();
+        } else {
+          hipLaunchKernelGGL((k_line_resid_sweep<B, T, LPR>), dim3(gr), dim3(256), 0, s, C->nlines, lc0, nlc, Lc,
+                             C->ln_len.p, C->ln_cell.p, S.rowptr.p, S.col.p, Af, V::yc(*C).p, r1l, V::r2(*C).p);
+          launch_check();
+          hipLaunchKernelGGL((k_line_gs<B, T, D, true>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s,
+                             C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
+                             V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, z, yc, yl);
+          launch_check();
+        }
+      }
     }
     return;
   }

@@ -111,6 +111,8 @@ class CPRSolver:
     reorthogonalize: bool = True
     verbose: int = 0
     local_stage: str = "line"
+    pressure_cycles: int = 1
+    local_sweeps: int = 4
     last_iterations: int = 0
     history: list = field(default_factory=list)
 
@@ -123,6 +125,8 @@ class CPRSolver:
         o.coarse_size, o.pairwise_passes = self.coarse_size, self.pairwise_passes
         o.reorthogonalize, o.verbose = int(self.reorthogonalize), self.verbose
         o.local_stage = {"block_jacobi": 0, "line": 1}[self.local_stage]
+        o.pressure_cycles = self.pressure_cycles
+        o.local_sweeps = self.local_sweeps
         return o
 
 
