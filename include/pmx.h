@@ -42,6 +42,7 @@ typedef struct pmx_solver_options {
   int32_t local_stage;      /* 0: cell block-Jacobi, 1: coloured line Gauss-Seidel */
   int32_t pressure_cycles;  /* V-cycles per pressure-stage application (stationary iteration) */
   int32_t local_sweeps;     /* Gauss-Seidel sweeps over the line colours */
+  int32_t use_graphs;       /* 1: replay the preconditioner as a captured HIP graph */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {

@@ -32,6 +32,7 @@ class SolverOptions(ctypes.Structure):
         ("local_stage", ctypes.c_int32),
         ("pressure_cycles", ctypes.c_int32),
         ("local_sweeps", ctypes.c_int32),
+        ("use_graphs", ctypes.c_int32),
     ]
 
 

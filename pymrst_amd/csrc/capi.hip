@@ -55,6 +55,7 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.local_stage = o->local_stage;
   r.amg_cycles = o->pressure_cycles;
   r.local_sweeps = o->local_sweeps;
+  r.use_graphs = o->use_graphs;
   return r;
 }
 
@@ -83,6 +84,7 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->local_stage = d.local_stage;
   o->pressure_cycles = d.amg_cycles;
   o->local_sweeps = d.local_sweeps;
+  o->use_graphs = d.use_graphs;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {

@@ -74,6 +74,7 @@ struct LinearSolverOptions {
   int verbose = 0;
   int local_stage = 1;       // 0: cell block-Jacobi; 1: coloured line Gauss-Seidel
   int local_sweeps = 4;      // Gauss-Seidel sweeps of the line local stage
+  int use_graphs = 1;        // replay the preconditioner application as a captured HIP graph
 };
 
 struct SolveInfo {

@@ -35,6 +35,31 @@ template struct DArr<unsigned char>;
 
 int grid_for(size_t threads, int block) { return (int)((threads + block - 1) / block); }
 
+// Input and output vectors of one preconditioner application, read by the
+// kernels from device memory so that a captured graph of the application
+// serves every Krylov column.
+struct PrecIO {
+  const double* r;
+  double* z;
+};
+
+__global__ void k_set_io(PrecIO* io, const double* r, double* z) {
+  io->r = r;
+  io->z = z;
+}
+
+template <class T>
+__global__ void k_fill(size_t n, T v, T* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+template <class T>
+__global__ void k_copy(size_t n, const T* __restrict__ a, T* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = a[i];
+}
+
 static inline void launch_check() { PMX_HIP(hipGetLastError()); }
 
 // ---------------------------------------------------------------------------
@@ -512,10 +537,11 @@ __global__ void k_tail_inverse(int nw, const double* __restrict__ Dp, double* __
 
 // r_p = W r (cell rows)
 template <int B, class T>
-__global__ void k_weight_restrict(int nc, const double* __restrict__ w, const double* __restrict__ r,
+__global__ void k_weight_restrict(int nc, const double* __restrict__ w, const PrecIO* __restrict__ io,
                                   T* __restrict__ rp) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
+  const double* __restrict__ r = io->r;
   double s = 0.0;
 #pragma unroll
   for (int a = 0; a < B; ++a) s += w[a] * r[(size_t)c * B + a];
@@ -526,8 +552,10 @@ __global__ void k_weight_restrict(int nc, const double* __restrict__ w, const do
 template <int B, class T, int LPR>
 __global__ __launch_bounds__(256) void k_cpr_stage2(int nc, const int* __restrict__ rowptr,
                                                     const int* __restrict__ col, const T* __restrict__ Jp,
-                                                    const T* __restrict__ xp, const double* __restrict__ r,
-                                                    const T* __restrict__ inv, double* __restrict__ z) {
+                                                    const T* __restrict__ xp, const PrecIO* __restrict__ io,
+                                                    const T* __restrict__ inv) {
+  const double* __restrict__ r = io->r;
+  double* __restrict__ z = io->z;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = t / LPR, lane = t % LPR;
   double acc[B];
@@ -562,8 +590,10 @@ template <int B, int M, int X, class T>
 __global__ __launch_bounds__(64) void k_cpr_wells(int nw, const int* __restrict__ well_ptr,
                                                   const int* __restrict__ perf_cell, const double* __restrict__ Cp,
                                                   const double* __restrict__ V, const double* __restrict__ Dinv,
-                                                  const T* __restrict__ xp, const double* __restrict__ r,
-                                                  size_t ncB, double* __restrict__ z, double* __restrict__ hp) {
+                                                  const T* __restrict__ xp, const PrecIO* __restrict__ io,
+                                                  size_t ncB, double* __restrict__ hp) {
+  const double* __restrict__ r = io->r;
+  double* __restrict__ z = io->z;
   const int w = blockIdx.x, lane = threadIdx.x;
   double ay[M], ah[X];
 #pragma unroll
@@ -603,9 +633,10 @@ template <int B, int X, class T>
 __global__ void k_cpr_mixfix(int npc, const int* __restrict__ pc_cell, const int* __restrict__ pc_ptr,
                              const int* __restrict__ pc_perf, const int* __restrict__ perf_well,
                              const double* __restrict__ Um, const double* __restrict__ hp,
-                             const T* __restrict__ inv, double* __restrict__ z) {
+                             const T* __restrict__ inv, const PrecIO* __restrict__ io) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= npc) return;
+  double* __restrict__ z = io->z;
   const int c = pc_cell[u];
   double acc[B];
 #pragma unroll
@@ -700,28 +731,31 @@ __global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __
 
 // Line Gauss-Seidel local stage.  The local system is A y = r1 on the cell
 // rows, r1 = r - J P xp - sum Um_j hp_w (the pressure correction and the
-// wells' mixture coupling of P xp removed); lines of one colour are solved
-// together and the colours are swept in order, several times if asked.
+// wells' mixture coupling of P xp removed).  Lines of one colour share no
+// matrix entry, so a colour's lines are solved together from
+//   A_line y_line = r1 - sum_{j off the line} A_cj y_j
+// with y_j the newest values: earlier colours of this sweep, later colours
+// of the previous sweep (zero in the first sweep).  The colours are swept in
+// order, `sweeps` times.
 //
-// First sweep, colour `color`: the right-hand side of every cell on a line
-// of this colour, written in line order (position-major):
-//   r2 = r1 - sum_{j of an earlier colour} J_cj y_j
-// where y holds the line solutions of the earlier colours (ycell, by cell).
-// r1 itself is kept in r1l when later sweeps need it; xpl receives xp in the
-// same order.  LPR lanes per row.
+// Right-hand sides of the cells on the lines of colour `color`, written in
+// line order (position-major), LPR lanes per row.  In the first sweep r1 is
+// formed here (and kept in r1l when later sweeps need it; xpl receives xp in
+// the same order); later sweeps read it back from r1l.
 template <int B, int X, class T, int LPR>
-__global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, int Lc, int color,
-                                                    const int* __restrict__ len, const int* __restrict__ cell,
-                                                    const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                    const T* __restrict__ Jp, const T* __restrict__ Af,
-                                                    const unsigned char* __restrict__ ccol,
-                                                    const T* __restrict__ xp, const T* __restrict__ ycell,
-                                                    const double* __restrict__ r, const int* __restrict__ pc_of,
-                                                    const int* __restrict__ pc_ptr, const int* __restrict__ pc_perf,
-                                                    const int* __restrict__ perf_well, const double* __restrict__ Um,
-                                                    const double* __restrict__ hp, T* __restrict__ r2l,
-                                                    T* __restrict__ r1l, T* __restrict__ xpl) {
+__global__ __launch_bounds__(256) void k_line_rhs(int nl, int lc0, int nlc, int Lc, int color, int first,
+                                                  const int* __restrict__ len, const int* __restrict__ cell,
+                                                  const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                  const T* __restrict__ Jp, const T* __restrict__ Af,
+                                                  const unsigned char* __restrict__ ccol,
+                                                  const T* __restrict__ xp, const T* __restrict__ ycell,
+                                                  const PrecIO* __restrict__ io, const int* __restrict__ pc_of,
+                                                  const int* __restrict__ pc_ptr, const int* __restrict__ pc_perf,
+                                                  const int* __restrict__ perf_well, const double* __restrict__ Um,
+                                                  const double* __restrict__ hp, T* __restrict__ r2l,
+                                                  T* __restrict__ r1l, T* __restrict__ xpl) {
   constexpr int BB = B * B;
+  const double* __restrict__ r = io->r;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t slot = t / LPR;
   const int lane = (int)(t % LPR);
@@ -737,10 +771,13 @@ __global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, in
     const int e = rowptr[row + 1];
     for (int q = rowptr[row] + lane; q < e; q += LPR) {
       const int j = col[q];
-      const double xj = (double)xp[j];
+      if (first) {
+        const double xj = (double)xp[j];
 #pragma unroll
-      for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
-      if (color > 0 && ccol[j] < color) {
+        for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
+      }
+      const int cj = ccol[j];
+      if (first ? cj < color : cj != color) {
 #pragma unroll
         for (int a = 0; a < B; ++a)
 #pragma unroll
@@ -756,95 +793,57 @@ __global__ __launch_bounds__(256) void k_line_resid(int nl, int lc0, int nlc, in
   }
   if (valid && lane == 0) {
     double r1[B];
+    if (first) {
 #pragma unroll
-    for (int a = 0; a < B; ++a) r1[a] = r[(size_t)row * B + a] - acc[a];
-    if (pc_of != nullptr) {
-      const int u = pc_of[row];
-      if (u >= 0)
-        for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
-          const int j = pc_perf[m], w = perf_well[j];
+      for (int a = 0; a < B; ++a) r1[a] = r[(size_t)row * B + a] - acc[a];
+      if (pc_of != nullptr) {
+        const int u = pc_of[row];
+        if (u >= 0)
+          for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
+            const int j = pc_perf[m], w = perf_well[j];
 #pragma unroll
-          for (int a = 0; a < B; ++a)
+            for (int a = 0; a < B; ++a)
 #pragma unroll
-            for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
-        }
+              for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
+          }
+      }
+      if (r1l != nullptr) {
+#pragma unroll
+        for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
+      }
+      xpl[at] = xp[row];
+    } else {
+#pragma unroll
+      for (int a = 0; a < B; ++a) r1[a] = (double)r1l[at * B + a];
     }
 #pragma unroll
     for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)(r1[a] - accy[a]);
-    if (r1l != nullptr) {
-#pragma unroll
-      for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
-    }
-    xpl[at] = xp[row];
-  }
-}
-
-// Later sweeps, colour `color`: r2 = r1 - A y over the whole row, with y the
-// current local solution of every cell (ycell).
-template <int B, class T, int LPR>
-__global__ __launch_bounds__(256) void k_line_resid_sweep(int nl, int lc0, int nlc, int Lc,
-                                                          const int* __restrict__ len, const int* __restrict__ cell,
-                                                          const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                          const T* __restrict__ Af, const T* __restrict__ ycell,
-                                                          const T* __restrict__ r1l, T* __restrict__ r2l) {
-  constexpr int BB = B * B;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t slot = t / LPR;
-  const int lane = (int)(t % LPR);
-  const int li = (int)(slot % nlc), k = (int)(slot / nlc);
-  const int l = lc0 + li;
-  const bool valid = k < Lc && k < len[l];
-  const size_t at = (size_t)k * nl + l;
-  const int row = valid ? cell[at] : 0;
-  double acc[B];
-#pragma unroll
-  for (int a = 0; a < B; ++a) acc[a] = 0.0;
-  if (valid) {
-    const int e = rowptr[row + 1];
-    for (int q = rowptr[row] + lane; q < e; q += LPR) {
-      const int j = col[q];
-      double yj[B];
-#pragma unroll
-      for (int b = 0; b < B; ++b) yj[b] = (double)ycell[(size_t)j * B + b];
-#pragma unroll
-      for (int a = 0; a < B; ++a)
-#pragma unroll
-        for (int b = 0; b < B; ++b) acc[a] = fma((double)Af[(size_t)q * BB + a * B + b], yj[b], acc[a]);
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < B; ++a) acc[a] = wave_sum(acc[a], LPR);
-  if (valid && lane == 0) {
-#pragma unroll
-    for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)((double)r1l[at * B + a] - acc[a]);
   }
 }
 
 // Block-tridiagonal solves of the lines of one colour, one thread per line:
 // forward y_k = D'_k^{-1} (r2_k - L_k y_{k-1}) (y overwrites r2 in place),
-// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}.  The first sweep's solution
-// is x; a later sweep's x is a correction added to the line's current
-// solution yl.  The new solution goes to yl and ycell when given, and
-// z = P xp + solution.  A wave walks the padded length of its longest line
-// (lines are sorted by length, tables are zero past each line's end and
-// every colour fills whole waves), so every load address is known without a
-// data-dependent index and the next D positions are in flight while the
-// recurrence runs; the scheduling barriers keep the compiler from sinking
-// those loads next to their use.
+// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}; then z = P xp + x and, when
+// ycell is given, y_cell = x for the colours and sweeps that follow.  A wave
+// walks the padded length of its longest line (lines are sorted by length,
+// tables are zero past each line's end and every colour fills whole waves),
+// so every load address is known without a data-dependent index and the
+// next D positions are in flight while the recurrence runs; the scheduling
+// barriers keep the compiler from sinking those loads next to their use.
 template <int B, class T>
 constexpr int line_depth() {
   return B * B * (int)sizeof(T) <= 16 ? 16 : (B * B * (int)sizeof(T) <= 36 ? 8 : 4);
 }
 
-template <int B, class T, int D, bool ACC>
+template <int B, class T, int D>
 __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __restrict__ len,
                                                 const int* __restrict__ cell, const T* __restrict__ inv,
                                                 const T* __restrict__ Lo, const T* __restrict__ Up,
                                                 T* __restrict__ r2l, const T* __restrict__ xpl,
-                                                double* __restrict__ z, T* __restrict__ ycell,
-                                                T* __restrict__ yl) {
+                                                const PrecIO* __restrict__ io, T* __restrict__ ycell) {
   static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
   constexpr int BB = B * B;
+  double* __restrict__ z = io->z;
   const int l = lc0 + blockIdx.x * blockDim.x + threadIdx.x;
   const int Lp = (len[lc0 + blockIdx.x * blockDim.x] + D - 1) / D * D;
   T ri[D][BB], rl[D][BB], rr[D][B];
@@ -889,9 +888,9 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  T bi[D][BB], bu[D][BB], by[D][B], bo[D][B], bx[D];
+  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
   int bc[D];
-  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T (&vo)[B], T& vx, int& c) {
+  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
     const size_t at = (size_t)k * nl + l;
 #pragma unroll
     for (int q = 0; q < BB; ++q) {
@@ -899,15 +898,12 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       vu[q] = Up[at * BB + q];
     }
 #pragma unroll
-    for (int a = 0; a < B; ++a) {
-      vy[a] = r2l[at * B + a];
-      if constexpr (ACC) vo[a] = yl[at * B + a];
-    }
+    for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
     vx = xpl[at];
     c = cell[at];
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bo[u], bx[u], bc[u]);
+  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
   double xn[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) xn[a] = 0.0;
@@ -930,23 +926,15 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
         for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
         xn[a] = v;
       }
-      double sol[B];
-#pragma unroll
-      for (int a = 0; a < B; ++a) sol[a] = ACC ? (double)bo[u][a] + xn[a] : xn[a];
-      if (yl != nullptr) {
-        const size_t at = (size_t)k * nl + l;
-#pragma unroll
-        for (int a = 0; a < B; ++a) yl[at * B + a] = (T)sol[a];
-      }
       const int c = bc[u];
       if (c >= 0) {
 #pragma unroll
         for (int a = 0; a < B; ++a) {
-          z[(size_t)c * B + a] = sol[a] + (a == 0 ? (double)bx[u] : 0.0);
-          if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)sol[a];
+          z[(size_t)c * B + a] = xn[a] + (a == 0 ? (double)bx[u] : 0.0);
+          if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
         }
       }
-      bload(max(k - D, 0), bi[u], bu[u], by[u], bo[u], bx[u], bc[u]);
+      bload(max(k - D, 0), bi[u], bu[u], by[u], bx[u], bc[u]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1111,8 +1099,19 @@ class CprSolver {
   std::vector<int> user_line_ptr, user_line_cells;   // lines given by the caller
   DArr<int> ln_len, ln_cell, ln_lo, ln_up;
   DArr<unsigned char> ln_color;                      // colour of each cell's line
-  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64, yl64;
-  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, yl32, A32;
+  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64;
+  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, A32;
+  // captured preconditioner application
+  PrecIO* io = nullptr;
+  hipStream_t cap_s = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_ok = false;
+  LinearSolverOptions graph_opts;
+  ~CprSolver() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (cap_s) (void)hipStreamDestroy(cap_s);
+    if (io) (void)hipFree(io);
+  }
   // repeated pressure cycles: original right-hand side and accumulated solution
   DArr<double> pb64, pacc64;
   DArr<float> pb32, pacc32;
@@ -1141,6 +1140,7 @@ void cpr_set_lines(CprSolver* C, int nlines, const int* ptr, const int* cells) {
   C->user_line_ptr.assign(ptr, ptr + nlines + 1);
   C->user_line_cells.assign(cells, cells + ptr[nlines]);
   C->built = false;   // rebuild the static structures with these lines
+  C->graph_ok = false;
 }
 
 void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<double>& weights,
@@ -1205,6 +1205,7 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   C->dense.alloc((size_t)C->coarse_n * C->coarse_n);
   PMX_HIP(hipStreamSynchronize(s));
   C->built = true;
+  C->graph_ok = false;
   C->prec_setup = 0;
   if (o.verbose) {
     fprintf(stderr, "[pmx] AMG hierarchy:");
@@ -1234,7 +1235,6 @@ struct Lv<double> {
   static DArr<double>& xpl(CprSolver& c) { return c.xpl64; }
   static DArr<double>& yc(CprSolver& c) { return c.yc64; }
   static DArr<double>& r1l(CprSolver& c) { return c.r1l64; }
-  static DArr<double>& yl(CprSolver& c) { return c.yl64; }
   static DArr<double>& pb(CprSolver& c) { return c.pb64; }
   static DArr<double>& pacc(CprSolver& c) { return c.pacc64; }
 };
@@ -1255,7 +1255,6 @@ struct Lv<float> {
   static DArr<float>& xpl(CprSolver& c) { return c.xpl32; }
   static DArr<float>& yc(CprSolver& c) { return c.yc32; }
   static DArr<float>& r1l(CprSolver& c) { return c.r1l32; }
-  static DArr<float>& yl(CprSolver& c) { return c.yl32; }
   static DArr<float>& pb(CprSolver& c) { return c.pb32; }
   static DArr<float>& pacc(CprSolver& c) { return c.pacc32; }
 };
@@ -1302,9 +1301,7 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::yc(*C).alloc((size_t)S.nc * B);
     if (o.local_sweeps > 1) {
       V::r1l(*C).alloc(pos * B);
-      V::yl(*C).alloc(pos * B);
       V::r1l(*C).zero(s);
-      V::yl(*C).zero(s);
     }
     // entries past each line's end stay zero: the solves run over padded lengths
     V::lf_inv(*C).zero(s);
@@ -1345,7 +1342,12 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
       launch_check();
     }
   }
+  if (o.amg_cycles > 1) {
+    V::pb(*C).alloc(f.n);
+    V::pacc(*C).alloc(f.n);
+  }
   C->prec_setup = sizeof(T) == 8 ? 64 : 32;
+  C->graph_ok = false;
   C->wdev = wdev;
 }
 
@@ -1401,27 +1403,26 @@ static void vcycle(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStre
 }
 
 template <int B, int M, int X, class T>
-static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
-                   hipStream_t s);
+static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions& o, hipStream_t s);
 
 template <int B, int M, int X, class T>
-static void apply_t(CprSolver* C, const BlockSystem& S, const double* r, double* z, const double* wdev,
-                    const LinearSolverOptions& o, hipStream_t s) {
+static void apply_t(CprSolver* C, const BlockSystem& S, const double* wdev, const LinearSolverOptions& o,
+                    hipStream_t s) {
   using V = Lv<T>;
   AmgLevel& f = *C->L[0];
   constexpr int LPR = 8;
   const size_t ncB = (size_t)S.nc * B;
   // stage 1: pressure
-  hipLaunchKernelGGL((k_weight_restrict<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, wdev, r, V::b(f).p);
+  hipLaunchKernelGGL((k_weight_restrict<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, wdev, C->io, V::b(f).p);
   launch_check();
   vcycle<T>(C, 0, o, s);
   if (o.amg_cycles > 1) {
     // stationary iteration with the V-cycle: x <- x + V(b - A_p x)
     DArr<T>& b0 = V::pb(*C);
     DArr<T>& acc = V::pacc(*C);
-    b0.alloc(f.n);
-    acc.alloc(f.n);
-    PMX_HIP(hipMemcpyAsync(b0.p, V::b(f).p, sizeof(T) * f.n, hipMemcpyDeviceToDevice, s));
+    PMX_CHECK(b0.n == (size_t)f.n && acc.n == (size_t)f.n, "pressure-cycle buffers are set up with the solver");
+    hipLaunchKernelGGL((k_copy<T>), dim3(grid_for(f.n)), dim3(256), 0, s, (size_t)f.n, V::b(f).p, b0.p);
+    launch_check();
     for (int cyc = 1; cyc < o.amg_cycles; ++cyc) {
       std::swap(V::x(f).p, acc.p);
       hipLaunchKernelGGL((k_csr_sweep<T, LPR>), dim3(grid_for((size_t)f.n * LPR)), dim3(256), 0, s, f.n,
@@ -1432,22 +1433,21 @@ static void apply_t(CprSolver* C, const BlockSystem& S, const double* r, double*
       launch_check();
     }
   }
-  stage2<B, M, X, T>(C, S, r, z, o, s);
+  stage2<B, M, X, T>(C, S, o, s);
 }
 
 // Stage 2: z = P xp + local(r - J P xp) with the local stage chosen by the
 // options (cell block-Jacobi, or line block-tridiagonal solves); the well
 // tail is solved exactly per well.
 template <int B, int M, int X, class T>
-static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
-                   hipStream_t s) {
+static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions& o, hipStream_t s) {
   using V = Lv<T>;
   AmgLevel& f = *C->L[0];
   constexpr int LPR = 8;
   const size_t ncB = (size_t)S.nc * B;
   if (S.nw > 0) {
     hipLaunchKernelGGL((k_cpr_wells<B, M, X, T>), dim3(S.nw), dim3(64), 0, s, S.nw, S.well_ptr.p, S.perf_cell.p,
-                       S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, r, ncB, z, C->hp.p);
+                       S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, C->io, ncB, C->hp.p);
     launch_check();
   }
   if (o.local_stage == 1 && C->nlines > 0) {
@@ -1457,68 +1457,102 @@ static void stage2(CprSolver* C, const BlockSystem& S, const double* r, double* 
     const int sweeps = std::max(1, o.local_sweeps);
     constexpr int D = line_depth<B, T>();
     T* r1l = sweeps > 1 ? V::r1l(*C).p : nullptr;
-    T* yl = sweeps > 1 ? V::yl(*C).p : nullptr;
     for (int sw = 0; sw < sweeps; ++sw) {
       for (int c = 0; c < C->ncolor; ++c) {
         const int lc0 = C->color_ptr[c], nlc = C->color_ptr[c + 1] - lc0, Lc = C->color_len[c];
         if (nlc == 0) continue;
         PMX_CHECK(nlc % LINE_GROUP == 0, "line colours must fill whole waves");
-        const int gr = grid_for((size_t)nlc * Lc * LPR);
-        T* yc = (sweeps > 1 || c + 1 < C->ncolor) ? V::yc(*C).p : nullptr;
-        if (sw == 0) {
-          hipLaunchKernelGGL((k_line_resid<B, X, T, LPR>), dim3(gr), dim3(256), 0, s, C->nlines, lc0, nlc, Lc, c,
-                             C->ln_len.p, C->ln_cell.p, S.rowptr.p, S.col.p, V::Jp(*C).p, Af, C->ln_color.p,
-                             V::x(f).p, V::yc(*C).p, r, mix ? S.pc_of.p : nullptr, S.pc_ptr.p, S.pc_perf.p,
-                             S.perf_well.p, S.Um.p, C->hp.p, V::r2(*C).p, r1l, V::xpl(*C).p);
-          launch_check();
-          hipLaunchKernelGGL((k_line_gs<B, T, D, false>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s,
-                             C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
-                             V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, z, yc, yl);
-          launch_check();
-        } else {
-          hipLaunchKernelGGL((k_line_resid_sweep<B, T, LPR>), dim3(gr), dim3(256), 0, s, C->nlines, lc0, nlc, Lc,
-                             C->ln_len.p, C->ln_cell.p, S.rowptr.p, S.col.p, Af, V::yc(*C).p, r1l, V::r2(*C).p);
-          launch_check();
-          hipLaunchKernelGGL((k_line_gs<B, T, D, true>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s,
-                             C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
-                             V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, z, yc, yl);
-          launch_check();
-        }
+        const bool last = sw + 1 == sweeps && c + 1 == C->ncolor;
+        hipLaunchKernelGGL((k_line_rhs<B, X, T, LPR>), dim3(grid_for((size_t)nlc * Lc * LPR)), dim3(256), 0, s,
+                           C->nlines, lc0, nlc, Lc, c, sw == 0 ? 1 : 0, C->ln_len.p, C->ln_cell.p, S.rowptr.p,
+                           S.col.p, V::Jp(*C).p, Af, C->ln_color.p, V::x(f).p, V::yc(*C).p, C->io,
+                           mix ? S.pc_of.p : nullptr, S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p,
+                           V::r2(*C).p, r1l, V::xpl(*C).p);
+        launch_check();
+        hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
+                           C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
+                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p);
+        launch_check();
       }
     }
     return;
   }
   hipLaunchKernelGGL((k_cpr_stage2<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, r, V::inv(*C).p, z);
+                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, V::inv(*C).p);
   launch_check();
   if (S.nw > 0 && S.has_mix && S.npc > 0) {
     hipLaunchKernelGGL((k_cpr_mixfix<B, X, T>), dim3(grid_for(S.npc)), dim3(256), 0, s, S.npc, S.pc_cell.p,
-                       S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::inv(*C).p, z);
+                       S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, V::inv(*C).p, C->io);
     launch_check();
   }
 }
 
 // Preconditioner without the pressure stage: block-Jacobi + tail only.
 template <int B, int M, int X, class T>
-static void apply_local_t(CprSolver* C, const BlockSystem& S, const double* r, double* z,
-                          const LinearSolverOptions& o, hipStream_t s) {
+static void apply_local_t(CprSolver* C, const BlockSystem& S, const LinearSolverOptions& o, hipStream_t s) {
   using V = Lv<T>;
   AmgLevel& f = *C->L[0];
-  PMX_HIP(hipMemsetAsync(V::x(f).p, 0, sizeof(T) * f.n, s));
-  stage2<B, M, X, T>(C, S, r, z, o, s);
+  hipLaunchKernelGGL((k_fill<T>), dim3(grid_for(f.n)), dim3(256), 0, s, (size_t)f.n, (T)0, V::x(f).p);
+  launch_check();
+  stage2<B, M, X, T>(C, S, o, s);
+}
+
+// The kernels of one application, reading r and z through C->io.
+static void apply_sequence(CprSolver* C, const BlockSystem& S, const LinearSolverOptions& o, hipStream_t s) {
+  if (o.precision == 32) {
+    if (o.use_pressure_stage) PMX_DISPATCH(S, (apply_t<B_, M_, X_, float>(C, S, C->wdev, o, s)));
+    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, float>(C, S, o, s)));
+  } else {
+    if (o.use_pressure_stage) PMX_DISPATCH(S, (apply_t<B_, M_, X_, double>(C, S, C->wdev, o, s)));
+    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, double>(C, S, o, s)));
+  }
+}
+
+static bool same_apply_options(const LinearSolverOptions& a, const LinearSolverOptions& b) {
+  return a.precision == b.precision && a.use_pressure_stage == b.use_pressure_stage &&
+         a.pre_smooth == b.pre_smooth && a.post_smooth == b.post_smooth && a.coarse_alpha == b.coarse_alpha &&
+         a.amg_cycles == b.amg_cycles && a.local_stage == b.local_stage && a.local_sweeps == b.local_sweeps;
+}
+
+// Capture the application once per setup (on a private stream: nothing runs
+// while capturing) and replay it; a replay costs one launch instead of one
+// per kernel.
+static void capture_apply(CprSolver* C, const BlockSystem& S, const LinearSolverOptions& o) {
+  if (C->cap_s == nullptr) PMX_HIP(hipStreamCreateWithFlags(&C->cap_s, hipStreamNonBlocking));
+  if (C->gexec != nullptr) {
+    PMX_HIP(hipGraphExecDestroy(C->gexec));
+    C->gexec = nullptr;
+  }
+  PMX_HIP(hipStreamBeginCapture(C->cap_s, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    apply_sequence(C, S, o, C->cap_s);
+  } catch (...) {
+    (void)hipStreamEndCapture(C->cap_s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  PMX_HIP(hipStreamEndCapture(C->cap_s, &g));
+  const hipError_t e = hipGraphInstantiate(&C->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  PMX_HIP(e);
+  C->graph_opts = o;
+  C->graph_ok = true;
 }
 
 void cpr_apply(CprSolver* C, const BlockSystem& S, const double* r, double* z, const LinearSolverOptions& o,
                hipStream_t s) {
   PMX_CHECK(C->prec_setup == o.precision, "preconditioner precision differs from its setup");
   PMX_CHECK(C->wdev != nullptr, "preconditioner weights unset");
-  if (o.precision == 32) {
-    if (o.use_pressure_stage) PMX_DISPATCH(S, (apply_t<B_, M_, X_, float>(C, S, r, z, C->wdev, o, s)));
-    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, float>(C, S, r, z, o, s)));
-  } else {
-    if (o.use_pressure_stage) PMX_DISPATCH(S, (apply_t<B_, M_, X_, double>(C, S, r, z, C->wdev, o, s)));
-    else PMX_DISPATCH(S, (apply_local_t<B_, M_, X_, double>(C, S, r, z, o, s)));
+  if (C->io == nullptr) PMX_HIP(hipMalloc(&C->io, sizeof(PrecIO)));
+  hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, s, C->io, r, z);
+  launch_check();
+  if (!o.use_graphs) {
+    apply_sequence(C, S, o, s);
+    return;
   }
+  if (!C->graph_ok || !same_apply_options(C->graph_opts, o)) capture_apply(C, S, o);
+  PMX_HIP(hipGraphLaunch(C->gexec, s));
 }
 
 

@@ -113,6 +113,7 @@ class CPRSolver:
     local_stage: str = "line"
     pressure_cycles: int = 1
     local_sweeps: int = 4
+    use_graphs: bool = True
     last_iterations: int = 0
     history: list = field(default_factory=list)
 
@@ -127,6 +128,7 @@ class CPRSolver:
         o.local_stage = {"block_jacobi": 0, "line": 1}[self.local_stage]
         o.pressure_cycles = self.pressure_cycles
         o.local_sweeps = self.local_sweeps
+        o.use_graphs = int(self.use_graphs)
         return o
 
 

@@ -179,3 +179,42 @@ def test_gravity_crossflow_schedule_matches_oracle(prec):
     sw = np.vstack([s.sW for s in states])
     swo = np.vstack([s.sW for s in ostates])
     assert np.abs(sw - swo).max() <= 1e-8
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_graph_replay_matches_eager(prec):
+    """The preconditioner replayed from its captured HIP graph gives the
+    eager sequence's result bit for bit, replay after replay, with eager
+    work and synchronisations between the replays and a different input
+    vector each time."""
+    import ctypes
+    from pymrst_amd import _lib
+    from pymrst_amd.ow import CPRSolver
+    inp = small_gravity_inputs()
+    mo, so = build_oracle(inp)
+    md, sd = build_device(inp, CPRSolver(rtol=1e-10, precision=prec))
+    st = _perturbed(so, 5, wells=False)
+    cdp = torch.as_tensor(np.concatenate(mo.compute_cdp_per_well(so)), device="cuda")
+    md.assemble_device(_dev_state(so), _dev_state(st), 10 * og.DAY, cdp)
+    ok, pmean = md.converged(10 * og.DAY, _dev_state(st))
+    opts = md.linear_solver.options()
+    info = _lib.SolveInfo()
+    assert md.lib.pmx_ow_solve(md.h, ctypes.byref(opts), pmean, _lib.ptr(md._dx), ctypes.byref(info)) == 0
+    n = md._dx.numel()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    rs = [torch.randn(n, dtype=torch.float64, device="cuda", generator=gen) for _ in range(3)]
+
+    def apply(r, graphs):
+        o = md.linear_solver.options()
+        o.use_graphs = int(graphs)
+        z = torch.full_like(r, float("nan"))
+        _lib.check(md.lib.pmx_ow_precondition(md.h, ctypes.byref(o), _lib.ptr(r), _lib.ptr(z)))
+        return z
+
+    eager = [apply(r, False).clone() for r in rs]
+    for rep in range(3):
+        for k, r in enumerate(rs):
+            z = apply(r, True)
+            torch.cuda.synchronize()
+            assert torch.equal(z, eager[k]), f"replay {rep}, vector {k}: differs from the eager result"
+            assert torch.equal(apply(r, False), eager[k])
