@@ -43,6 +43,7 @@ typedef struct pmx_solver_options {
   int32_t pressure_cycles;  /* V-cycles per pressure-stage application (stationary iteration) */
   int32_t local_sweeps;     /* Gauss-Seidel sweeps over the line colours */
   int32_t use_graphs;       /* 1: replay the preconditioner as a captured HIP graph */
+  int32_t coarse_block_rows; /* AMG levels up to this many rows run in one workgroup */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {

@@ -33,6 +33,7 @@ class SolverOptions(ctypes.Structure):
         ("pressure_cycles", ctypes.c_int32),
         ("local_sweeps", ctypes.c_int32),
         ("use_graphs", ctypes.c_int32),
+        ("coarse_block_rows", ctypes.c_int32),
     ]
 
 

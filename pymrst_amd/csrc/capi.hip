@@ -56,6 +56,7 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.amg_cycles = o->pressure_cycles;
   r.local_sweeps = o->local_sweeps;
   r.use_graphs = o->use_graphs;
+  r.coarse_block_rows = o->coarse_block_rows;
   return r;
 }
 
@@ -85,6 +86,7 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->pressure_cycles = d.amg_cycles;
   o->local_sweeps = d.local_sweeps;
   o->use_graphs = d.use_graphs;
+  o->coarse_block_rows = d.coarse_block_rows;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {

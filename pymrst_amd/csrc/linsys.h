@@ -75,6 +75,7 @@ struct LinearSolverOptions {
   int local_stage = 1;       // 0: cell block-Jacobi; 1: coloured line Gauss-Seidel
   int local_sweeps = 4;      // Gauss-Seidel sweeps of the line local stage
   int use_graphs = 1;        // replay the preconditioner application as a captured HIP graph
+  int coarse_block_rows = 0;     // AMG levels up to this size run in one workgroup (the coarsest always does)
 };
 
 struct SolveInfo {

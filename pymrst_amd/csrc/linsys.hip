@@ -32,6 +32,7 @@ template struct DArr<double>;
 template struct DArr<float>;
 template struct DArr<int>;
 template struct DArr<unsigned char>;
+template struct DArr<char>;
 
 int grid_for(size_t threads, int block) { return (int)((threads + block - 1) / block); }
 
@@ -312,9 +313,64 @@ void system_matvec(const BlockSystem& S, const double* x, double* y, double* h, 
 // ---------------------------------------------------------------------------
 // Vector kernels (FP64, deterministic two-stage reductions)
 // ---------------------------------------------------------------------------
-constexpr int kRedBlocks = 1024;   // partial-sum blocks for reductions
-constexpr int kRedThreads = 256;
+constexpr int kRedThreads = 256;   // reductions: threads per block
+constexpr int kRedElems = 4;       // elements per thread (a block covers one tile)
+constexpr int kRedTile = kRedThreads * kRedElems;
+constexpr int kRedGroup = 8;       // dot products formed per pass over a tile
 
+static inline int red_blocks(size_t n) { return (int)((n + kRedTile - 1) / kRedTile); }
+
+// partial[blk * nd + d] = sum over tile blk of V_d . w  (d < nv) and w.w
+// (d == nv).  w stays in registers; each group of kRedGroup dots keeps
+// kRedGroup * kRedElems loads in flight per thread.
+__global__ __launch_bounds__(kRedThreads) void k_multidot(size_t n, int nv, const double* __restrict__ V,
+                                                          size_t ld, const double* __restrict__ w,
+                                                          double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads / 64][kRedGroup];
+  const int nd = nv + 1;
+  const size_t base = (size_t)blockIdx.x * kRedTile + threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double wv[kRedElems];
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    const size_t i = base + (size_t)e * kRedThreads;
+    wv[e] = i < n ? w[i] : 0.0;
+  }
+  for (int d0 = 0; d0 < nd; d0 += kRedGroup) {
+    double acc[kRedGroup];
+#pragma unroll
+    for (int q = 0; q < kRedGroup; ++q) {
+      const int d = d0 + q;
+      double a = 0.0;
+      if (d < nv) {
+#pragma unroll
+        for (int e = 0; e < kRedElems; ++e) {
+          const size_t i = base + (size_t)e * kRedThreads;
+          if (i < n) a = fma(V[(size_t)d * ld + i], wv[e], a);
+        }
+      } else if (d == nv) {
+#pragma unroll
+        for (int e = 0; e < kRedElems; ++e) a = fma(wv[e], wv[e], a);
+      }
+      acc[q] = wave_sum(a, 64);
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < kRedGroup; ++q) sh[wid][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < kRedGroup && d0 + (int)threadIdx.x < nd) {
+      double t = 0.0;
+#pragma unroll
+      for (int v = 0; v < kRedThreads / 64; ++v) t += sh[v][threadIdx.x];
+      partial[(size_t)blockIdx.x * nd + d0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// out[d] = sum_blk partial[blk*nd + d]: one 256-wide block per d, fixed
+// order within each thread's stride and a fixed shuffle tree, so the result
+// does not depend on timing.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   v = wave_sum(v, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -327,38 +383,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;
 }
 
-// partial[blk * nd + d] = sum over the block's range of V_d . w  (d < nv) and w.w (d == nv)
-__global__ __launch_bounds__(kRedThreads) void k_multidot(size_t n, int nv, const double* __restrict__ V,
-                                                          size_t ld, const double* __restrict__ w,
-                                                          double* __restrict__ partial) {
-  __shared__ double sh[kRedThreads / 64];
-  const int nd = nv + 1;
-  const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-  // process dots in groups of 4 to reuse the w loads
-  for (int d0 = 0; d0 < nd; d0 += 4) {
-    double acc[4] = {0, 0, 0, 0};
-    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-      const double wi = w[i];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int d = d0 + q;
-        if (d < nv) acc[q] = fma(V[(size_t)d * ld + i], wi, acc[q]);
-        else if (d == nv) acc[q] = fma(wi, wi, acc[q]);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int d = d0 + q;
-      double s = block_sum(acc[q], sh);
-      if (threadIdx.x == 0 && d < nd) partial[(size_t)blockIdx.x * nd + d] = s;
-    }
-  }
-}
-
-// out[d] = sum_blk partial[blk*nd + d]: one 256-wide block per d, fixed
-// order within each thread's stride and a fixed shuffle tree, so the result
-// does not depend on timing.
 __global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int nd, const double* __restrict__ partial,
                                                          double* __restrict__ out) {
   __shared__ double sh[4];
@@ -370,19 +394,34 @@ __global__ __launch_bounds__(256) void k_reduce_partials(int nblk, int nd, const
   if (threadIdx.x == 0) out[d] = s;
 }
 
-// w -= sum_d coef[d] V_d ; partial[blk] = sum of w^2 over the block's range.
+// w -= sum_d coef[d] V_d ; partial[blk] = sum of w^2 over tile blk.
 __global__ __launch_bounds__(kRedThreads) void k_multiaxpy_norm(size_t n, int nv, const double* __restrict__ V,
                                                                 size_t ld, const double* __restrict__ coef,
                                                                 double* __restrict__ w, double* __restrict__ partial) {
   __shared__ double sh[kRedThreads / 64];
-  const size_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  const size_t base = (size_t)blockIdx.x * kRedTile + threadIdx.x;
+  double wv[kRedElems];
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    const size_t i = base + (size_t)e * kRedThreads;
+    wv[e] = i < n ? w[i] : 0.0;
+  }
+  for (int d = 0; d < nv; ++d) {
+    const double c = coef[d];
+#pragma unroll
+    for (int e = 0; e < kRedElems; ++e) {
+      const size_t i = base + (size_t)e * kRedThreads;
+      if (i < n) wv[e] = fma(-c, V[(size_t)d * ld + i], wv[e]);
+    }
+  }
   double acc = 0.0;
-  for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    double wi = w[i];
-    for (int d = 0; d < nv; ++d) wi = fma(-coef[d], V[(size_t)d * ld + i], wi);
-    w[i] = wi;
-    acc = fma(wi, wi, acc);
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    const size_t i = base + (size_t)e * kRedThreads;
+    if (i < n) {
+      w[i] = wv[e];
+      acc = fma(wv[e], wv[e], acc);
+    }
   }
   double s = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -424,14 +463,14 @@ __global__ void k_axpby(size_t n, double a, const double* __restrict__ x, double
 }
 
 double device_dot(const double* a, const double* b, size_t n, double* partial, hipStream_t s) {
-  // partial needs kRedBlocks*2 + 2 doubles
-  hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, 1, a, n, b, partial);
+  // partial needs red_blocks(n)*2 + 2 doubles
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_multidot, dim3(nb), dim3(kRedThreads), 0, s, n, 1, a, n, b, partial);
   launch_check();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(2), dim3(256), 0, s, kRedBlocks, 2, partial,
-                     partial + (size_t)kRedBlocks * 2);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(2), dim3(256), 0, s, nb, 2, partial, partial + (size_t)nb * 2);
   launch_check();
   double h[2];
-  PMX_HIP(hipMemcpyAsync(h, partial + (size_t)kRedBlocks * 2, sizeof(h), hipMemcpyDeviceToHost, s));
+  PMX_HIP(hipMemcpyAsync(h, partial + (size_t)nb * 2, sizeof(h), hipMemcpyDeviceToHost, s));
   PMX_HIP(hipStreamSynchronize(s));
   return h[0];
 }
@@ -442,7 +481,7 @@ double device_dot(const double* a, const double* b, size_t n, double* partial, h
 // Pressure matrix A_p = W J P and the pressure column of J (per block).
 template <int B, class T>
 __global__ void k_pressure_matrix(int nnzb, const double* __restrict__ val, const double* __restrict__ w,
-                                  T* __restrict__ Ap, T* __restrict__ Jp, T* __restrict__ Afull) {
+                                  T* __restrict__ Ap, T* __restrict__ Jp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnzb) return;
   double s = 0.0;
@@ -453,10 +492,6 @@ __global__ void k_pressure_matrix(int nnzb, const double* __restrict__ val, cons
     Jp[(size_t)k * B + a] = (T)v;
   }
   Ap[k] = (T)s;
-  if (Afull != nullptr) {
-#pragma unroll
-    for (int q = 0; q < B * B; ++q) Afull[(size_t)k * B * B + q] = (T)val[(size_t)k * B * B + q];
-  }
 }
 
 // Small dense inverse (Gauss-Jordan with partial pivoting) in registers.
@@ -738,87 +773,98 @@ __global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __
 // of the previous sweep (zero in the first sweep).  The colours are swept in
 // order, `sweeps` times.
 //
-// Right-hand sides of the cells on the lines of colour `color`, written in
-// line order (position-major), LPR lanes per row.  In the first sweep r1 is
-// formed here (and kept in r1l when later sweeps need it; xpl receives xp in
-// the same order); later sweeps read it back from r1l.
+// r1 on every cell row (LPR lanes per row), scattered to the row's line
+// position; xpl receives xp in the same order.
 template <int B, int X, class T, int LPR>
-__global__ __launch_bounds__(256) void k_line_rhs(int nl, int lc0, int nlc, int Lc, int color, int first,
-                                                  const int* __restrict__ len, const int* __restrict__ cell,
-                                                  const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                  const T* __restrict__ Jp, const T* __restrict__ Af,
-                                                  const unsigned char* __restrict__ ccol,
-                                                  const T* __restrict__ xp, const T* __restrict__ ycell,
-                                                  const PrecIO* __restrict__ io, const int* __restrict__ pc_of,
-                                                  const int* __restrict__ pc_ptr, const int* __restrict__ pc_perf,
-                                                  const int* __restrict__ perf_well, const double* __restrict__ Um,
-                                                  const double* __restrict__ hp, T* __restrict__ r2l,
-                                                  T* __restrict__ r1l, T* __restrict__ xpl) {
-  constexpr int BB = B * B;
+__global__ __launch_bounds__(256) void k_line_r1(int nc, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                 const T* __restrict__ Jp, const T* __restrict__ xp,
+                                                 const PrecIO* __restrict__ io, const int* __restrict__ pc_of,
+                                                 const int* __restrict__ pc_ptr, const int* __restrict__ pc_perf,
+                                                 const int* __restrict__ perf_well, const double* __restrict__ Um,
+                                                 const double* __restrict__ hp, const int* __restrict__ pos,
+                                                 T* __restrict__ r1l, T* __restrict__ xpl) {
   const double* __restrict__ r = io->r;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t slot = t / LPR;
-  const int lane = (int)(t % LPR);
-  const int li = (int)(slot % nlc), k = (int)(slot / nlc);
-  const int l = lc0 + li;
-  const bool valid = k < Lc && k < len[l];
-  const size_t at = (size_t)k * nl + l;
-  const int row = valid ? cell[at] : 0;
-  double acc[B], accy[B];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = t / LPR, lane = t % LPR;
+  double acc[B];
 #pragma unroll
-  for (int a = 0; a < B; ++a) acc[a] = accy[a] = 0.0;
-  if (valid) {
+  for (int a = 0; a < B; ++a) acc[a] = 0.0;
+  if (row < nc) {
     const int e = rowptr[row + 1];
     for (int q = rowptr[row] + lane; q < e; q += LPR) {
-      const int j = col[q];
-      if (first) {
-        const double xj = (double)xp[j];
+      const double xj = (double)xp[col[q]];
 #pragma unroll
-        for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
-      }
-      const int cj = ccol[j];
-      if (first ? cj < color : cj != color) {
-#pragma unroll
-        for (int a = 0; a < B; ++a)
-#pragma unroll
-          for (int b = 0; b < B; ++b)
-            accy[a] = fma((double)Af[(size_t)q * BB + a * B + b], (double)ycell[(size_t)j * B + b], accy[a]);
-      }
+      for (int a = 0; a < B; ++a) acc[a] = fma((double)Jp[(size_t)q * B + a], xj, acc[a]);
     }
   }
 #pragma unroll
-  for (int a = 0; a < B; ++a) {
-    acc[a] = wave_sum(acc[a], LPR);
-    accy[a] = wave_sum(accy[a], LPR);
-  }
-  if (valid && lane == 0) {
+  for (int a = 0; a < B; ++a) acc[a] = wave_sum(acc[a], LPR);
+  if (row < nc && lane == 0) {
     double r1[B];
-    if (first) {
 #pragma unroll
-      for (int a = 0; a < B; ++a) r1[a] = r[(size_t)row * B + a] - acc[a];
-      if (pc_of != nullptr) {
-        const int u = pc_of[row];
-        if (u >= 0)
-          for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
-            const int j = pc_perf[m], w = perf_well[j];
+    for (int a = 0; a < B; ++a) r1[a] = r[(size_t)row * B + a] - acc[a];
+    if (pc_of != nullptr) {
+      const int u = pc_of[row];
+      if (u >= 0)
+        for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
+          const int j = pc_perf[m], w = perf_well[j];
 #pragma unroll
-            for (int a = 0; a < B; ++a)
+          for (int a = 0; a < B; ++a)
 #pragma unroll
-              for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
-          }
-      }
-      if (r1l != nullptr) {
-#pragma unroll
-        for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
-      }
-      xpl[at] = xp[row];
-    } else {
-#pragma unroll
-      for (int a = 0; a < B; ++a) r1[a] = (double)r1l[at * B + a];
+            for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
+        }
     }
+    const size_t at = (size_t)pos[row];
 #pragma unroll
-    for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)(r1[a] - accy[a]);
+    for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
+    xpl[at] = xp[row];
   }
+}
+
+// Right-hand sides of the lines of colour `color` (one thread per line
+// position): r2 = r1 - sum_h Aoff_h y_nbr_h over the couplings off the line;
+// in the first sweep only neighbours of earlier colours have a value yet.
+template <int B, class T>
+__global__ __launch_bounds__(256) void k_line_rhs(int nl, int lc0, int nlc, int Lc, int color, int first, int noff,
+                                                  size_t npos, const int* __restrict__ len,
+                                                  const int* __restrict__ off_nbr,
+                                                  const unsigned char* __restrict__ off_col,
+                                                  const T* __restrict__ aoff, const T* __restrict__ ycell,
+                                                  const T* __restrict__ r1l, T* __restrict__ r2l) {
+  constexpr int BB = B * B;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int li = (int)(t % nlc), k = (int)(t / nlc);
+  const int l = lc0 + li;
+  if (k >= Lc || k >= len[l]) return;
+  const size_t at = (size_t)k * nl + l;
+  double r2[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) r2[a] = (double)r1l[at * B + a];
+  for (int h = 0; h < noff; ++h) {
+    const size_t e = (size_t)h * npos + at;
+    const int j = off_nbr[e];
+    if (j < 0 || (first && off_col[e] >= color)) continue;
+    double yj[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) yj[b] = (double)ycell[(size_t)j * B + b];
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+      for (int b = 0; b < B; ++b) r2[a] -= (double)aoff[e * BB + a * B + b] * yj[b];
+  }
+#pragma unroll
+  for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)r2[a];
+}
+
+// Off-line coupling blocks in table order (one thread per plane entry).
+template <int B, class T>
+__global__ void k_gather_off(size_t ne, const int* __restrict__ off_q, const double* __restrict__ val,
+                             T* __restrict__ aoff) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int q = off_q[e];
+#pragma unroll
+  for (int m = 0; m < B * B; ++m) aoff[e * B * B + m] = q >= 0 ? (T)val[(size_t)q * B * B + m] : (T)0;
 }
 
 // Block-tridiagonal solves of the lines of one colour, one thread per line:
@@ -1067,6 +1113,92 @@ __global__ void k_dense_solve(int n, const double* __restrict__ ainv, const T* _
   x[i] = (T)s;
 }
 
+// The coarse end of the V-cycle in one workgroup: levels small enough that
+// a launch per smoothing sweep would cost more than the sweep.  Level k of
+// the description is the k-th of these levels; the last one is solved with
+// the dense inverse.  Same cycle as the launched levels: x = D^-1 b, further
+// Jacobi sweeps, residual restricted to the coarser level, coarse
+// correction scaled by alpha, post-smoothing sweeps.
+template <class T>
+struct CoarseLevel {
+  int n;
+  const int *rowptr, *col, *agg, *agg_ptr, *agg_mem;
+  const T *v, *d;
+  T *b, *x, *y;
+};
+
+constexpr int kCoarseThreads = 1024;
+constexpr int kMaxCoarseLevels = 16;
+
+template <class T>
+__device__ void cb_sweep(const CoarseLevel<T>& a, const T* __restrict__ x, T* __restrict__ y) {
+  for (int i = threadIdx.x; i < a.n; i += blockDim.x) {
+    double acc = 0.0;
+    for (int k = a.rowptr[i]; k < a.rowptr[i + 1]; ++k) acc += (double)a.v[k] * (double)x[a.col[k]];
+    y[i] = (T)((double)x[i] + (double)a.d[i] * ((double)a.b[i] - acc));
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(int nlev, const CoarseLevel<T>* __restrict__ L,
+                                                                 int pre, int post, double alpha,
+                                                                 const double* __restrict__ dense) {
+  __shared__ T* xs[kMaxCoarseLevels];
+  const int tid = threadIdx.x;
+  for (int l = 0; l + 1 < nlev; ++l) {
+    const CoarseLevel<T> a = L[l];
+    T* x = a.x;
+    T* y = a.y;
+    for (int i = tid; i < a.n; i += blockDim.x) x[i] = a.d[i] * a.b[i];
+    __syncthreads();
+    for (int it = 1; it < pre; ++it) {
+      cb_sweep(a, x, y);
+      __syncthreads();
+      T* t = x; x = y; y = t;
+    }
+    if (tid == 0) xs[l] = x;
+    const CoarseLevel<T> c = L[l + 1];
+    for (int I = tid; I < c.n; I += blockDim.x) {
+      double sI = 0.0;
+      for (int m = a.agg_ptr[I]; m < a.agg_ptr[I + 1]; ++m) {
+        const int i = a.agg_mem[m];
+        double acc = 0.0;
+        for (int k = a.rowptr[i]; k < a.rowptr[i + 1]; ++k) acc += (double)a.v[k] * (double)x[a.col[k]];
+        sI += (double)a.b[i] - acc;
+      }
+      c.b[I] = (T)sI;
+    }
+    __syncthreads();
+  }
+  {
+    const CoarseLevel<T> a = L[nlev - 1];
+    for (int i = tid; i < a.n; i += blockDim.x) {
+      double sI = 0.0;
+      for (int j = 0; j < a.n; ++j) sI += dense[(size_t)i * a.n + j] * (double)a.b[j];
+      a.x[i] = (T)sI;
+    }
+    if (tid == 0) xs[nlev - 1] = a.x;
+    __syncthreads();
+  }
+  for (int l = nlev - 2; l >= 0; --l) {
+    const CoarseLevel<T> a = L[l];
+    T* x = xs[l];
+    T* y = x == a.x ? a.y : a.x;
+    const T* xc = L[l + 1].x;   // level l+1 leaves its result in its own x
+    for (int i = tid; i < a.n; i += blockDim.x) x[i] = (T)((double)x[i] + alpha * (double)xc[a.agg[i]]);
+    __syncthreads();
+    for (int it = 0; it < post; ++it) {
+      cb_sweep(a, x, y);
+      __syncthreads();
+      T* t = x; x = y; y = t;
+    }
+    if (x != a.x) {
+      for (int i = tid; i < a.n; i += blockDim.x) a.x[i] = x[i];
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // CprSolver
 // ---------------------------------------------------------------------------
@@ -1099,8 +1231,14 @@ class CprSolver {
   std::vector<int> user_line_ptr, user_line_cells;   // lines given by the caller
   DArr<int> ln_len, ln_cell, ln_lo, ln_up;
   DArr<unsigned char> ln_color;                      // colour of each cell's line
-  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64;
-  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, A32;
+  int noff = 0;                                      // off-line coupling planes
+  DArr<int> ln_pos, off_q, off_nbr;
+  DArr<unsigned char> off_col;
+  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64, aoff64;
+  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, aoff32;
+  // coarse levels handled by one workgroup: first such level, descriptions
+  int coarse_l0 = -1;
+  DArr<char> coarse_desc;
   // captured preconditioner application
   PrecIO* io = nullptr;
   hipStream_t cap_s = nullptr;
@@ -1200,6 +1338,11 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   C->ln_lo.upload(HL.lo, s);
   C->ln_up.upload(HL.up, s);
   C->ln_color.upload(HL.cell_color, s);
+  C->noff = HL.noff;
+  C->ln_pos.upload(HL.pos, s);
+  C->off_q.upload(HL.off_q, s);
+  C->off_nbr.upload(HL.off_nbr, s);
+  C->off_col.upload(HL.off_col, s);
   C->coarse_n = C->L.back()->n;
   PMX_CHECK(C->coarse_n <= 4096, "coarsest AMG level is too large for the dense solve");
   C->dense.alloc((size_t)C->coarse_n * C->coarse_n);
@@ -1235,6 +1378,7 @@ struct Lv<double> {
   static DArr<double>& xpl(CprSolver& c) { return c.xpl64; }
   static DArr<double>& yc(CprSolver& c) { return c.yc64; }
   static DArr<double>& r1l(CprSolver& c) { return c.r1l64; }
+  static DArr<double>& aoff(CprSolver& c) { return c.aoff64; }
   static DArr<double>& pb(CprSolver& c) { return c.pb64; }
   static DArr<double>& pacc(CprSolver& c) { return c.pacc64; }
 };
@@ -1255,6 +1399,7 @@ struct Lv<float> {
   static DArr<float>& xpl(CprSolver& c) { return c.xpl32; }
   static DArr<float>& yc(CprSolver& c) { return c.yc32; }
   static DArr<float>& r1l(CprSolver& c) { return c.r1l32; }
+  static DArr<float>& aoff(CprSolver& c) { return c.aoff32; }
   static DArr<float>& pb(CprSolver& c) { return c.pb32; }
   static DArr<float>& pacc(CprSolver& c) { return c.pacc32; }
 };
@@ -1278,15 +1423,8 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
   DArr<T>& inv = V::inv(*C);
   Jp.alloc((size_t)S.nnzb * B);
   inv.alloc((size_t)S.nc * B * B);
-  T* Afull = nullptr;
-  if constexpr (std::is_same<T, float>::value) {
-    if (o.local_stage == 1 && C->nlines > 0) {
-      C->A32.alloc((size_t)S.nnzb * B * B);
-      Afull = C->A32.p;
-    }
-  }
   hipLaunchKernelGGL((k_pressure_matrix<B, T>), dim3(grid_for(S.nnzb)), dim3(256), 0, s, S.nnzb, S.val.p, wdev,
-                     V::v(f).p, Jp.p, Afull);
+                     V::v(f).p, Jp.p);
   launch_check();
   hipLaunchKernelGGL((k_block_inverse<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, S.diag.p, S.val.p,
                      inv.p);
@@ -1299,9 +1437,14 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::r2(*C).alloc(pos * B);
     V::xpl(*C).alloc(pos);
     V::yc(*C).alloc((size_t)S.nc * B);
-    if (o.local_sweeps > 1) {
-      V::r1l(*C).alloc(pos * B);
-      V::r1l(*C).zero(s);
+    V::r1l(*C).alloc(pos * B);
+    V::r1l(*C).zero(s);
+    const size_t ne = (size_t)C->noff * pos;
+    V::aoff(*C).alloc(std::max<size_t>(ne, 1) * B * B);
+    if (ne > 0) {
+      hipLaunchKernelGGL((k_gather_off<B, T>), dim3(grid_for(ne)), dim3(256), 0, s, ne, C->off_q.p, S.val.p,
+                         V::aoff(*C).p);
+      launch_check();
     }
     // entries past each line's end stay zero: the solves run over padded lengths
     V::lf_inv(*C).zero(s);
@@ -1346,6 +1489,31 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::pb(*C).alloc(f.n);
     V::pacc(*C).alloc(f.n);
   }
+  // the coarse end of the cycle: the longest tail of levels with at most
+  // coarse_block_rows rows each
+  const int nL = (int)C->L.size();
+  int l0 = nL - 1;
+  while (l0 > 0 && C->L[l0 - 1]->n <= o.coarse_block_rows) --l0;
+  PMX_CHECK(nL - l0 <= kMaxCoarseLevels, "too many coarse levels");
+  std::vector<CoarseLevel<T>> desc(nL - l0);
+  for (int l = l0; l < nL; ++l) {
+    AmgLevel& a = *C->L[l];
+    CoarseLevel<T>& d = desc[l - l0];
+    d.n = a.n;
+    d.rowptr = a.rowptr.p;
+    d.col = a.col.p;
+    d.agg = a.agg.p;
+    d.agg_ptr = a.agg_ptr.p;
+    d.agg_mem = a.agg_mem.p;
+    d.v = V::v(a).p;
+    d.d = V::d(a).p;
+    d.b = V::b(a).p;
+    d.x = V::x(a).p;
+    d.y = V::y(a).p;
+  }
+  C->coarse_desc.upload(reinterpret_cast<const char*>(desc.data()), sizeof(CoarseLevel<T>) * desc.size(), s);
+  C->coarse_l0 = l0;
+  PMX_HIP(hipStreamSynchronize(s));
   C->prec_setup = sizeof(T) == 8 ? 64 : 32;
   C->graph_ok = false;
   C->wdev = wdev;
@@ -1366,6 +1534,13 @@ static void vcycle(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStre
   AmgLevel& a = *C->L[l];
   constexpr int LPR = 8;
   const int n = a.n;
+  if ((int)l == C->coarse_l0) {
+    hipLaunchKernelGGL((k_coarse_cycle<T>), dim3(1), dim3(kCoarseThreads), 0, s, (int)C->L.size() - (int)l,
+                       reinterpret_cast<const CoarseLevel<T>*>(C->coarse_desc.p), o.pre_smooth, o.post_smooth,
+                       o.coarse_alpha, C->dense.p);
+    launch_check();
+    return;
+  }
   if (l + 1 == C->L.size()) {
     hipLaunchKernelGGL((k_dense_solve<T>), dim3(grid_for(n)), dim3(256), 0, s, n, C->dense.p, V::b(a).p,
                        V::x(a).p);
@@ -1451,23 +1626,23 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
     launch_check();
   }
   if (o.local_stage == 1 && C->nlines > 0) {
-    const T* Af;
-    if constexpr (std::is_same<T, double>::value) Af = S.val.p; else Af = C->A32.p;
     const bool mix = S.nw > 0 && S.has_mix && S.npc > 0;
     const int sweeps = std::max(1, o.local_sweeps);
     constexpr int D = line_depth<B, T>();
-    T* r1l = sweeps > 1 ? V::r1l(*C).p : nullptr;
+    const size_t npos = (size_t)C->maxlen * C->nlines;
+    hipLaunchKernelGGL((k_line_r1<B, X, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                       S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, mix ? S.pc_of.p : nullptr, S.pc_ptr.p,
+                       S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, C->ln_pos.p, V::r1l(*C).p, V::xpl(*C).p);
+    launch_check();
     for (int sw = 0; sw < sweeps; ++sw) {
       for (int c = 0; c < C->ncolor; ++c) {
         const int lc0 = C->color_ptr[c], nlc = C->color_ptr[c + 1] - lc0, Lc = C->color_len[c];
         if (nlc == 0) continue;
         PMX_CHECK(nlc % LINE_GROUP == 0, "line colours must fill whole waves");
         const bool last = sw + 1 == sweeps && c + 1 == C->ncolor;
-        hipLaunchKernelGGL((k_line_rhs<B, X, T, LPR>), dim3(grid_for((size_t)nlc * Lc * LPR)), dim3(256), 0, s,
-                           C->nlines, lc0, nlc, Lc, c, sw == 0 ? 1 : 0, C->ln_len.p, C->ln_cell.p, S.rowptr.p,
-                           S.col.p, V::Jp(*C).p, Af, C->ln_color.p, V::x(f).p, V::yc(*C).p, C->io,
-                           mix ? S.pc_of.p : nullptr, S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p,
-                           V::r2(*C).p, r1l, V::xpl(*C).p);
+        hipLaunchKernelGGL((k_line_rhs<B, T>), dim3(grid_for((size_t)nlc * Lc)), dim3(256), 0, s, C->nlines, lc0,
+                           nlc, Lc, c, sw == 0 ? 1 : 0, C->noff, npos, C->ln_len.p, C->off_nbr.p, C->off_col.p,
+                           V::aoff(*C).p, V::yc(*C).p, V::r1l(*C).p, V::r2(*C).p);
         launch_check();
         hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
                            C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
@@ -1511,7 +1686,8 @@ static void apply_sequence(CprSolver* C, const BlockSystem& S, const LinearSolve
 static bool same_apply_options(const LinearSolverOptions& a, const LinearSolverOptions& b) {
   return a.precision == b.precision && a.use_pressure_stage == b.use_pressure_stage &&
          a.pre_smooth == b.pre_smooth && a.post_smooth == b.post_smooth && a.coarse_alpha == b.coarse_alpha &&
-         a.amg_cycles == b.amg_cycles && a.local_stage == b.local_stage && a.local_sweeps == b.local_sweeps;
+         a.amg_cycles == b.amg_cycles && a.local_stage == b.local_stage && a.local_sweeps == b.local_sweeps &&
+         a.coarse_block_rows == b.coarse_block_rows;
 }
 
 // Capture the application once per setup (on a private stream: nothing runs
@@ -1572,7 +1748,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     C->Z.alloc(ld * m);
     C->w.alloc(ld);
     C->r.alloc(ld);
-    C->partial.alloc((size_t)kRedBlocks * (m + 2) + 2 * (m + 2));
+    C->partial.alloc((size_t)red_blocks(n) * (m + 2) + 2 * (m + 2));
     C->hdev.alloc(m + 2);
     C->ydev.alloc(m + 2);
     C->hmat.alloc((size_t)std::max(S.nw, 1) * kMaxMix);
@@ -1582,7 +1758,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   }
   double* hscr = C->hmat.p;
   const int nb = grid_for(n);
-  double* red = C->partial.p + (size_t)kRedBlocks * (m + 2);
+  const int rb = red_blocks(n);
   // ||b||
   const double bnorm = std::sqrt(std::max(0.0, device_dot(b, b, n, C->partial.p, s)));
   PMX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
@@ -1622,10 +1798,9 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       // the projected vector follows from Pythagoras, and a second pass runs
       // only when it shows heavy cancellation ("twice is enough").
       const int nd = j + 2;
-      hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w,
-                         C->partial.p);
+      hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
       launch_check();
-      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, kRedBlocks, nd, C->partial.p, C->hdev.p);
+      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, rb, nd, C->partial.p, C->hdev.p);
       launch_check();
       std::vector<double> hh(nd, 0.0);
       PMX_HIP(hipMemcpyAsync(hh.data(), C->hdev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
@@ -1636,13 +1811,12 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       const double* coef = C->hdev.p;
       for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
       if (o.reorthogonalize && !(wn2 > 0.5 * hh[j + 1])) {
-        hipLaunchKernelGGL(k_multiaxpy_norm, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld,
-                           C->hdev.p, w, C->partial.p);
+        hipLaunchKernelGGL(k_multiaxpy_norm, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, C->hdev.p,
+                           w, C->partial.p);
         launch_check();
-        hipLaunchKernelGGL(k_multidot, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w,
-                           C->partial.p);
+        hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
         launch_check();
-        hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, kRedBlocks, nd, C->partial.p, C->ydev.p);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, rb, nd, C->partial.p, C->ydev.p);
         launch_check();
         std::vector<double> h2(nd, 0.0);
         PMX_HIP(hipMemcpyAsync(h2.data(), C->ydev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));

@@ -98,10 +98,16 @@ void build_aggregation(const HostPattern& fine, const std::vector<double>& w,
 // maxlen is padded to a multiple of LINE_PAD (tables hold maxlen rows).
 constexpr int LINE_PAD = 16;
 constexpr int LINE_GROUP = 64;
+// The couplings of each line cell to cells off its line are listed in noff
+// position-major planes (plane h, entry h*maxlen*nlines + k*nlines + l):
+// the BCSR entry, the neighbour cell and the neighbour's colour (-1 / 255
+// where a cell has fewer).  pos[c] is the table position of cell c.
 struct HostLines {
-  int nlines = 0, maxlen = 0, longest = 0, ncolor = 0;
-  std::vector<int> len, cell, lo, up, color_ptr;
+  int nlines = 0, maxlen = 0, longest = 0, ncolor = 0, noff = 0;
+  std::vector<int> len, cell, lo, up, color_ptr, pos;
   std::vector<unsigned char> cell_color;   // colour of each cell's line
+  std::vector<int> off_q, off_nbr;
+  std::vector<unsigned char> off_col;
 };
 void build_lines(const HostPattern& P, const std::vector<double>& w, HostLines& L);
 void lines_from_list(const HostPattern& P, int nlines, const int* ptr, const int* cells, HostLines& L);

@@ -347,6 +347,37 @@ static void line_tables(const HostPattern& P, std::vector<std::vector<int>>& lin
       if (k + 1 < ln.size()) L.up[at] = entry(ln[k], ln[k + 1]);
     }
   }
+  // couplings off the line, per position
+  const size_t npos = (size_t)maxlen * nl;
+  L.pos.assign(n, -1);
+  std::vector<int> lof(n, -1);
+  for (int l = 0; l < nl; ++l)
+    for (size_t k = 0; k < sorted[l].size(); ++k) {
+      L.pos[sorted[l][k]] = (int)(k * nl + l);
+      lof[sorted[l][k]] = l;
+    }
+  int noff = 0;
+  for (int c = 0; c < n; ++c) {
+    int m = 0;
+    for (int q = P.rowptr[c]; q < P.rowptr[c + 1]; ++q) m += lof[P.col[q]] != lof[c];
+    noff = std::max(noff, m);
+  }
+  L.noff = noff;
+  L.off_q.assign((size_t)noff * npos, -1);
+  L.off_nbr.assign((size_t)noff * npos, -1);
+  L.off_col.assign((size_t)noff * npos, 255);
+  for (int c = 0; c < n; ++c) {
+    const size_t at = (size_t)L.pos[c];
+    int h = 0;
+    for (int q = P.rowptr[c]; q < P.rowptr[c + 1]; ++q) {
+      const int j = P.col[q];
+      if (lof[j] == lof[c]) continue;
+      L.off_q[h * npos + at] = q;
+      L.off_nbr[h * npos + at] = j;
+      L.off_col[h * npos + at] = L.cell_color[j];
+      ++h;
+    }
+  }
 }
 
 // Lines of strongly coupled cells: each cell links to its (at most) two

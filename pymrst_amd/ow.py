@@ -114,6 +114,7 @@ class CPRSolver:
     pressure_cycles: int = 1
     local_sweeps: int = 4
     use_graphs: bool = True
+    coarse_block_rows: int = 0
     last_iterations: int = 0
     history: list = field(default_factory=list)
 
@@ -129,6 +130,7 @@ class CPRSolver:
         o.pressure_cycles = self.pressure_cycles
         o.local_sweeps = self.local_sweeps
         o.use_graphs = int(self.use_graphs)
+        o.coarse_block_rows = self.coarse_block_rows
         return o
 
 
