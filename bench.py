@@ -26,6 +26,12 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_bcsr_spmv<2,8> launch on the 60x220x85 Jacobian, from the
+# FETCH_SIZE / WRITE_SIZE counter runs of `bench.py --traffic-probe 5`,
+# calibrated against a 512 MiB copy (fetch factor 2.0, write factor 1.0):
+# 404.3 MB read + 17.95 MB written.
+MEASURED_SPMV_TRAFFIC = {(60, 220, 85): 422.2e6}
+
 
 def spmv_bytes(nnzb: int, nc: int) -> float:
     """Algorithmic bytes of one 2x2-block BCSR SpMV launch: every block once
@@ -98,6 +104,35 @@ def cpu_baseline(case_sample, budget_s=25.0):
                         f"same workload; {wall:.1f} s, 1 thread of {cores} available"))
 
 
+def traffic_probe(case, reps):
+    """A short program for hardware-counter runs: the FP64 BCSR SpMV of the
+    assembled Jacobian `reps` times, then a plain 512 MiB device copy `reps`
+    times (the copy's known bytes calibrate the FETCH_SIZE / WRITE_SIZE
+    counters)."""
+    import ctypes
+    import torch
+    from pymrst_amd import _lib
+    from pymrst_amd.ow import CPRSolver, DeviceOWState
+    from pymrst_amd.workloads import build_model
+    model, s0 = build_model(case, CPRSolver(precision=32))
+    d0 = DeviceOWState.from_host(s0, model.device)
+    cdp = model.compute_cdp_per_well(d0)
+    model.assemble_device(d0, d0, float(case.dts[0]), cdp)
+    n = model._dx.numel()
+    x = torch.randn(n, dtype=torch.float64, device=model.device)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        _lib.check(model.lib.pmx_ow_matvec(model.h, _lib.ptr(x), _lib.ptr(y)), "pmx_ow_matvec")
+    a = torch.ones(1 << 26, dtype=torch.float64, device=model.device)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "spmv+copy", "reps": reps, "spmv_bytes": spmv_bytes(model.nnzb, case.num_cells),
+                      "copy_read_bytes": a.numel() * 8, "copy_write_bytes": b.numel() * 8}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +142,8 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--dims", default="60,220,85")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
+                    help="run only the SpMV and a calibration copy REPS times (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
@@ -119,6 +156,8 @@ def main():
     from pymrst_amd.workloads import field_waterflood
     dims = tuple(int(v) for v in args.dims.split(","))
     case = field_waterflood(dims=dims)
+    if args.traffic_probe > 0:
+        return traffic_probe(case, args.traffic_probe)
     steps = max(1, min(args.steps, case.dts.size))
     nc = case.num_cells
     forms = {"both": (64, 32), "mixed": (32,), "fp64": (64,)}[args.form]
@@ -149,7 +188,7 @@ def main():
         avg_s = prof["spmv_ms"] / prof["spmv_count"] / 1e3
         achieved = nbytes / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_SPMV_TRAFFIC.get(dims),
                 "kernel": "k_bcsr_spmv<2,8> (FP64 2x2 BCSR SpMV of the Jacobian)",
                 "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
                 "launches": prof["spmv_count"]}
