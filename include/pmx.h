@@ -134,6 +134,28 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4);
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);
 
+/* ---- discrete adjoint (pymrst/modules/optimization/adjoint.py:73-225) */
+/* Per connection f (host, 2*nconn): T_f/T_h at N0 and at N1, the derivative
+ * of the face transmissibility with respect to the log-permeability of each
+ * side, divided by T_f (dT_f/dln k_c = T_f^2/T_h,c; solvers/compute_trans.py
+ * and get_face_transmissibility.py give T_h and T_f). */
+int pmx_ow_set_trans_fractions(pmx_ow* h, const double* frac);
+/* (dR/dx)^T lam = rhs for the Jacobian of the last pmx_ow_assemble
+ * (adjoint.py:203 `spsolve(a.T, rhs)`), by CPR-FGMRES on the transposed
+ * system; rhs and lam are device vectors in the device unknown order.
+ * Returns 0 converged, 1 not converged, <0 error. */
+int pmx_ow_adjoint_solve(pmx_ow* h, const pmx_solver_options* o, double pmean, const double* rhs, double* lam,
+                         pmx_solve_info* info);
+/* out = (dR_n/dx_{n-1})^T lam (adjoint.py:85-107, accumulation only; zero
+ * on the well unknowns); p0, s0 are the previous state (device). */
+int pmx_ow_prev_state_transpose(pmx_ow* h, const double* p0, const double* s0, double dt, const double* lam,
+                                double* out);
+/* grad[c] += (dR/dln k_c)^T lam at the state (p, s, qws, qos, bhp) with the
+ * connection head cdp (device; nullptr: the handle's), through the face
+ * transmissibilities and the Peaceman well indices. */
+int pmx_ow_dlogk(pmx_ow* h, const double* p, const double* s, const double* qws, const double* qos,
+                 const double* bhp, const double* cdp, const double* lam, double* grad);
+
 /* ---- incompressible TPFA pressure solve (incomp_tpfa_ad,
  *      pymrst/modules/incomp/incomp_tpfa.py:252-389) and computeTrans
  *      (pymrst/solvers/compute_trans.py:11-29) */

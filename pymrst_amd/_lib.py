@@ -75,6 +75,11 @@ SIGNATURES = {
     "pmx_ow_profile": (ctypes.c_int, [_P, _I]),
     "pmx_ow_profile_read": (ctypes.c_int, [_P, _P]),
     "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
+    "pmx_ow_set_trans_fractions": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_adjoint_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), ctypes.c_double, _P, _P,
+                                            ctypes.POINTER(SolveInfo)]),
+    "pmx_ow_prev_state_transpose": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
+    "pmx_ow_dlogk": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "pmx_incomp_create": (ctypes.c_int, [_I, _P, ctypes.POINTER(_P)]),
     "pmx_incomp_destroy": (ctypes.c_int, [_P]),
     "pmx_incomp_set_grid": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P]),

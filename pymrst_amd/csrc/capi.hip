@@ -259,6 +259,8 @@ int pmx_ow_set_lines(pmx_ow* h, int32_t nlines, const int32_t* line_ptr, const i
   PMX_TRY({
     PMX_CHECK(h && line_ptr && cells && nlines >= 0, "null argument");
     cpr_set_lines(h->m->cpr, nlines, line_ptr, cells);
+    h->m->line_ptr.assign(line_ptr, line_ptr + nlines + 1);
+    h->m->line_cells.assign(cells, cells + line_ptr[nlines]);
   })
 }
 
@@ -376,6 +378,61 @@ int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r,
     bind(h);
     LinearSolverOptions lo = to_opts(o ? o : &h->last_opts);
     cpr_apply(h->m->cpr, h->m->S, r, z, lo, h->m->stream);
+  })
+}
+
+int pmx_ow_set_trans_fractions(pmx_ow* h, const double* frac) {
+  PMX_TRY({
+    PMX_CHECK(h && frac, "null argument");
+    bind(h);
+    h->m->set_trans_fractions(frac);
+  })
+}
+
+int pmx_ow_adjoint_solve(pmx_ow* h, const pmx_solver_options* o, double pmean, const double* rhs, double* lam,
+                         pmx_solve_info* info) {
+  try {
+    PMX_CHECK(h && rhs && lam, "null argument");
+    bind(h);
+    LinearSolverOptions lo = to_opts(o ? o : &h->last_opts);
+    SolveInfo si;
+    const int st = h->m->adjoint_solve(lo, pmean, rhs, lam, si);
+    if (info) {
+      info->iterations = si.iterations;
+      info->restarts = si.restarts;
+      info->rel_residual = si.rel_residual;
+      info->setup_ms = si.setup_ms;
+      info->solve_ms = si.solve_ms;
+      info->levels = si.levels;
+      info->coarse_n = si.coarse_n;
+    }
+    if (st < 0) {
+      g_err = "adjoint solve produced a non-finite residual";
+      return -2;
+    }
+    return st;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int pmx_ow_prev_state_transpose(pmx_ow* h, const double* p0, const double* s0, double dt, const double* lam,
+                                double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && p0 && s0 && lam && out && dt > 0.0, "bad argument");
+    bind(h);
+    h->m->prev_state_transpose(p0, s0, dt, lam, out);
+  })
+}
+
+int pmx_ow_dlogk(pmx_ow* h, const double* p, const double* s, const double* qws, const double* qos,
+                 const double* bhp, const double* cdp, const double* lam, double* grad) {
+  PMX_TRY({
+    PMX_CHECK(h && p && s && lam && grad, "null argument");
+    PMX_CHECK(h->m->nw == 0 || (qws && qos && bhp), "well state required");
+    bind(h);
+    h->m->dlogk(p, s, qws, qos, bhp, cdp, lam, grad);
   })
 }
 

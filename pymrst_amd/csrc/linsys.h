@@ -102,6 +102,10 @@ void cpr_set_lines(CprSolver*, int nlines, const int* ptr, const int* cells);
 // Numeric setup for the current values of S (pressure weights: w_dev[B]).
 void cpr_setup(CprSolver*, const BlockSystem& S, const double* w_dev, const LinearSolverOptions& o,
                hipStream_t s);
+// Same, with prolongation weights pc_dev (B doubles on the device) spreading
+// the pressure correction over a block's unknowns; nullptr means e_0.
+void cpr_setup(CprSolver*, const BlockSystem& S, const double* w_dev, const double* pc_dev,
+               const LinearSolverOptions& o, hipStream_t s);
 // Solve J x = b (FGMRES); returns 0 converged, 1 not converged.
 int cpr_solve(CprSolver*, const BlockSystem& S, const double* b, double* x, const LinearSolverOptions& o,
               SolveInfo& info, hipStream_t s);

@@ -478,16 +478,21 @@ double device_dot(const double* a, const double* b, size_t n, double* partial, h
 // ---------------------------------------------------------------------------
 // CPR preconditioner pieces
 // ---------------------------------------------------------------------------
-// Pressure matrix A_p = W J P and the pressure column of J (per block).
+// Pressure matrix A_p = W J P and the prolonged column J P (per block).
+// W combines the rows of a block with weights w; P spreads the pressure
+// unknown over the block's columns with weights pc (the pressure column,
+// pc = e_0, for the forward system).
 template <int B, class T>
 __global__ void k_pressure_matrix(int nnzb, const double* __restrict__ val, const double* __restrict__ w,
-                                  T* __restrict__ Ap, T* __restrict__ Jp) {
+                                  const double* __restrict__ pc, T* __restrict__ Ap, T* __restrict__ Jp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnzb) return;
   double s = 0.0;
 #pragma unroll
   for (int a = 0; a < B; ++a) {
-    const double v = val[(size_t)k * B * B + a * B];
+    double v = 0.0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) v = fma(val[(size_t)k * B * B + a * B + b], pc[b], v);
     s += w[a] * v;
     Jp[(size_t)k * B + a] = (T)v;
   }
@@ -588,7 +593,7 @@ template <int B, class T, int LPR>
 __global__ __launch_bounds__(256) void k_cpr_stage2(int nc, const int* __restrict__ rowptr,
                                                     const int* __restrict__ col, const T* __restrict__ Jp,
                                                     const T* __restrict__ xp, const PrecIO* __restrict__ io,
-                                                    const T* __restrict__ inv) {
+                                                    const T* __restrict__ inv, const double* __restrict__ pc) {
   const double* __restrict__ r = io->r;
   double* __restrict__ z = io->z;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -612,7 +617,7 @@ __global__ __launch_bounds__(256) void k_cpr_stage2(int nc, const int* __restric
     for (int a = 0; a < B; ++a) r2[a] = r[(size_t)row * B + a] - acc[a];
 #pragma unroll
     for (int a = 0; a < B; ++a) {
-      double s = (a == 0) ? (double)xp[row] : 0.0;
+      double s = pc[a] * (double)xp[row];
 #pragma unroll
       for (int b = 0; b < B; ++b) s = fma((double)inv[(size_t)row * B * B + a * B + b], r2[b], s);
       z[(size_t)row * B + a] = s;
@@ -626,7 +631,8 @@ __global__ __launch_bounds__(64) void k_cpr_wells(int nw, const int* __restrict_
                                                   const int* __restrict__ perf_cell, const double* __restrict__ Cp,
                                                   const double* __restrict__ V, const double* __restrict__ Dinv,
                                                   const T* __restrict__ xp, const PrecIO* __restrict__ io,
-                                                  size_t ncB, double* __restrict__ hp) {
+                                                  size_t ncB, double* __restrict__ hp,
+                                                  const double* __restrict__ pc) {
   const double* __restrict__ r = io->r;
   double* __restrict__ z = io->z;
   const int w = blockIdx.x, lane = threadIdx.x;
@@ -638,9 +644,13 @@ __global__ __launch_bounds__(64) void k_cpr_wells(int nw, const int* __restrict_
   for (int j = well_ptr[w] + lane; j < well_ptr[w + 1]; j += 64) {
     const double xc = (double)xp[perf_cell[j]];
 #pragma unroll
-    for (int q = 0; q < M; ++q) ay[q] += Cp[(size_t)j * M * B + q * B] * xc;
+    for (int q = 0; q < M; ++q)
 #pragma unroll
-    for (int a = 0; a < X; ++a) ah[a] += V[(size_t)j * X * B + a * B] * xc;
+      for (int b = 0; b < B; ++b) ay[q] += Cp[(size_t)j * M * B + q * B + b] * pc[b] * xc;
+#pragma unroll
+    for (int a = 0; a < X; ++a)
+#pragma unroll
+      for (int b = 0; b < B; ++b) ah[a] += V[(size_t)j * X * B + a * B + b] * pc[b] * xc;
   }
 #pragma unroll
   for (int q = 0; q < M; ++q) ay[q] = wave_sum(ay[q], 64);
@@ -886,10 +896,14 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
                                                 const int* __restrict__ cell, const T* __restrict__ inv,
                                                 const T* __restrict__ Lo, const T* __restrict__ Up,
                                                 T* __restrict__ r2l, const T* __restrict__ xpl,
-                                                const PrecIO* __restrict__ io, T* __restrict__ ycell) {
+                                                const PrecIO* __restrict__ io, T* __restrict__ ycell,
+                                                const double* __restrict__ pcw) {
   static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
   constexpr int BB = B * B;
   double* __restrict__ z = io->z;
+  double pc[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) pc[a] = pcw[a];
   const int l = lc0 + blockIdx.x * blockDim.x + threadIdx.x;
   const int Lp = (len[lc0 + blockIdx.x * blockDim.x] + D - 1) / D * D;
   T ri[D][BB], rl[D][BB], rr[D][B];
@@ -976,7 +990,7 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       if (c >= 0) {
 #pragma unroll
         for (int a = 0; a < B; ++a) {
-          z[(size_t)c * B + a] = xn[a] + (a == 0 ? (double)bx[u] : 0.0);
+          z[(size_t)c * B + a] = xn[a] + pc[a] * (double)bx[u];
           if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
         }
       }
@@ -1223,7 +1237,9 @@ class CprSolver {
   int m_alloc = 0;
   size_t n_alloc = 0, ld = 0;
   int prec_setup = 0;
-  const double* wdev = nullptr;       // pressure weights of the current setup
+  const double* wdev = nullptr;       // pressure weights of the current setup (row combination)
+  const double* pcdev = nullptr;      // prolongation weights of the current setup (column spread)
+  DArr<double> pc_unit;               // e_0: the pressure column
   EventTimer spmv_t, prec_t;
   // line local stage: position-major line tables and block-tridiagonal factors
   int nlines = 0, maxlen = 0, longest = 0, ncolor = 0;
@@ -1424,7 +1440,7 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
   Jp.alloc((size_t)S.nnzb * B);
   inv.alloc((size_t)S.nc * B * B);
   hipLaunchKernelGGL((k_pressure_matrix<B, T>), dim3(grid_for(S.nnzb)), dim3(256), 0, s, S.nnzb, S.val.p, wdev,
-                     V::v(f).p, Jp.p);
+                     C->pcdev, V::v(f).p, Jp.p);
   launch_check();
   hipLaunchKernelGGL((k_block_inverse<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, S.diag.p, S.val.p,
                      inv.p);
@@ -1521,6 +1537,19 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
 
 void cpr_setup(CprSolver* C, const BlockSystem& S, const double* w_dev, const LinearSolverOptions& o,
                hipStream_t s) {
+  cpr_setup(C, S, w_dev, nullptr, o, s);
+}
+
+void cpr_setup(CprSolver* C, const BlockSystem& S, const double* w_dev, const double* pc_dev,
+               const LinearSolverOptions& o, hipStream_t s) {
+  if (pc_dev == nullptr) {
+    if (C->pc_unit.n == 0) {
+      const double e0[4] = {1.0, 0.0, 0.0, 0.0};
+      C->pc_unit.upload(e0, 4, s);
+    }
+    pc_dev = C->pc_unit.p;
+  }
+  C->pcdev = pc_dev;
   if (o.precision == 32)
     PMX_DISPATCH(S, (setup_t<B_, M_, X_, float>(C, S, w_dev, o, s)));
   else
@@ -1622,7 +1651,7 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
   const size_t ncB = (size_t)S.nc * B;
   if (S.nw > 0) {
     hipLaunchKernelGGL((k_cpr_wells<B, M, X, T>), dim3(S.nw), dim3(64), 0, s, S.nw, S.well_ptr.p, S.perf_cell.p,
-                       S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, C->io, ncB, C->hp.p);
+                       S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, C->io, ncB, C->hp.p, C->pcdev);
     launch_check();
   }
   if (o.local_stage == 1 && C->nlines > 0) {
@@ -1646,14 +1675,14 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
         launch_check();
         hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
                            C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
-                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p);
+                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
         launch_check();
       }
     }
     return;
   }
   hipLaunchKernelGGL((k_cpr_stage2<B, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, V::inv(*C).p);
+                     S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, V::inv(*C).p, C->pcdev);
   launch_check();
   if (S.nw > 0 && S.has_mix && S.npc > 0) {
     hipLaunchKernelGGL((k_cpr_mixfix<B, X, T>), dim3(grid_for(S.npc)), dim3(256), 0, s, S.npc, S.pc_cell.p,

@@ -26,6 +26,15 @@ class OWModelDev {
   void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
               double* qos, double* bhp);
 
+  // adjoint (adjoint.hip): transposed solve of the last assembled Jacobian,
+  // the previous-state coupling and the log-permeability derivative
+  void set_trans_fractions(const double* frac);
+  void transpose_system();
+  int adjoint_solve(const LinearSolverOptions& o, double pmean, const double* rhs, double* lam, SolveInfo& info);
+  void prev_state_transpose(const double* p0, const double* s0, double dt, const double* lam, double* out);
+  void dlogk(const double* p, const double* s, const double* qws, const double* qos, const double* bhp,
+             const double* cdp_dev, const double* lam, double* grad);
+
   hipStream_t stream;
   OWFluid fluid{};
   int nc = 0, nconn = 0, nw = 0;
@@ -37,7 +46,13 @@ class OWModelDev {
   std::vector<int> host_types;
   DArr<double> r, rhs, w_cpr, conv_partial;
   std::vector<double> hier_weights;
+  std::vector<int> host_bconn, line_ptr, line_cells;
   CprSolver* cpr = nullptr;
+  // adjoint
+  BlockSystem ST;
+  DArr<int> tmap;                      // block k = (i,j) -> block (j,i)
+  DArr<double> bfrac, w_adj, gperf;    // per block: T_f / T_h of the row's half-face
+  CprSolver* cprT = nullptr;
 };
 
 }  // namespace pmx

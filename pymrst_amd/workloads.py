@@ -81,3 +81,33 @@ def build_model(case: OWCase, linear_solver=None, device=None):
     model = ow.TwoPhaseOilWaterModelAD(g, rock, fluid, wells, gravity=case.gravity,
                                        linear_solver=linear_solver, device=device)
     return model, model.initial_state(case.p0, case.sw0)
+
+
+def adjoint_problem(dims=(64, 64, 16), cell=(6.1, 3.05, 0.61), seed=4, steps=20, step_days=15.0,
+                    inj_rate=2.5e-3, prod_bhp=150 * BARSA, sigma=1.0, solver=None, checkpoint_every=1):
+    """The log-permeability adjoint case: oil-water with the field case's
+    fluids on a `dims` grid, ln k = ln(100 mD) + sigma z (z seeded standard
+    normal, kz/kx 0.1), porosity 0.2, a centre rate injector (2.5e-3 m^3/s,
+    about half a pore volume over the schedule at 64x64x16) and four corner
+    producers at 150 bar, no gravity, `steps` steps of `step_days` days.
+    Objective: sum over report steps of the producers' squared water-rate
+    mismatch against seeded targets (one quarter of the injection rate, 20 %
+    seeded spread).  Returns (problem, ln k)."""
+    from . import grid as G
+    from . import ow
+    from .adjoint import AdjointProblem
+    nx, ny, nz = dims
+    nc = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    logk = np.log(100 * MILLI * DARCY) + sigma * rng.standard_normal(nc)
+    g = G.cart_grid(dims, (nx * cell[0], ny * cell[1], nz * cell[2]))
+    column = lambda i, j: i + nx * j + nx * ny * np.arange(nz)
+    specs = [dict(cells=column(nx // 2, ny // 2), type="rate", val=inj_rate, compi=[1.0, 0.0], sign=1)]
+    for i, j in ((0, 0), (nx - 1, 0), (0, ny - 1), (nx - 1, ny - 1)):
+        specs.append(dict(cells=column(i, j), type="bhp", val=prod_bhp, compi=[0.0, 1.0], sign=-1))
+    targets = -0.25 * inj_rate * (1.0 + 0.2 * rng.standard_normal((steps, 4)))
+    fluid = ow.init_simple_adi_fluid((0.3 * CENTI * POISE, 3.0 * CENTI * POISE), (1000.0, 850.0), (2.0, 2.0),
+                                     (1e-9, 1e-9), 200 * BARSA, c_r=1e-10)
+    prob = AdjointProblem(g, np.full(nc, 0.2), fluid, specs, np.full(steps, step_days * DAY), 200 * BARSA, 0.2,
+                          solver=solver, targets=targets, checkpoint_every=checkpoint_every)
+    return prob, logk
