@@ -104,6 +104,49 @@ def cpu_baseline(case_sample, budget_s=25.0):
                         f"same workload; {wall:.1f} s, 1 thread of {cores} available"))
 
 
+def adjoint_leg(fd_cells: int = 0, seed: int = 5):
+    """The log-permeability adjoint on the 64x64x16 oil-water case (20 steps
+    of 15 days): one forward run keeping checkpoints, one backward sweep,
+    optionally central finite differences (h = 1e-6) on `fd_cells` seeded
+    random cells."""
+    import torch
+    from pymrst_amd.workloads import adjoint_problem
+    prob, logk = adjoint_problem()
+    prob.forward(logk)          # warm-up: library, allocations
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run = prob.forward(logk)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g, gu = prob.gradient(run)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    J = prob.value(run)
+    out = {"workload": "oil-water 64x64x16 (65,536 cells), 20 x 15 d, d J / d ln k for every cell",
+           "objective": J, "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+           "gradients_per_s": round(1.0 / (t2 - t0), 3), "newton_its": int(sum(run.newton)),
+           "adjoint_fgmres_its": int(sum(prob.adjoint_its))}
+    if fd_cells > 0:
+        rng = np.random.default_rng(seed)
+        cells = rng.choice(logk.size, size=fd_cells, replace=False)
+        gh = g.cpu().numpy()
+        fd = np.zeros(fd_cells)
+        h = 1e-6
+        for i, c in enumerate(cells):
+            lp, lm = logk.copy(), logk.copy()
+            lp[c] += h
+            lm[c] -= h
+            fd[i] = (prob.value(prob.forward(lp)) - prob.value(prob.forward(lm))) / (2 * h)
+            print(f"[bench] adjoint FD cell {int(c)}: adjoint {gh[c]:.9e} FD {fd[i]:.9e}", file=sys.stderr,
+                  flush=True)
+        adj = gh[cells]
+        out["fd_check"] = {"cells": fd_cells, "h": h,
+                           "max_abs_err_over_max_fd": float(np.abs(adj - fd).max() / np.abs(fd).max()),
+                           "max_rel_err_per_cell": float((np.abs(adj - fd) / np.abs(fd)).max()),
+                           "median_rel_err_per_cell": float(np.median(np.abs(adj - fd) / np.abs(fd)))}
+    return out
+
+
 def traffic_probe(case, reps):
     """A short program for hardware-counter runs: the FP64 BCSR SpMV of the
     assembled Jacobian `reps` times, then a plain 512 MiB device copy `reps`
@@ -142,6 +185,10 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--dims", default="60,220,85")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-fd", type=int, default=0, metavar="CELLS",
+                    help="check the adjoint gradient by central differences on CELLS random cells")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
                     help="run only the SpMV and a calibration copy REPS times (for counter runs)")
     args = ap.parse_args()
@@ -158,6 +205,9 @@ def main():
     case = field_waterflood(dims=dims)
     if args.traffic_probe > 0:
         return traffic_probe(case, args.traffic_probe)
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(args.adjoint_fd)}))
+        return 0
     steps = max(1, min(args.steps, case.dts.size))
     nc = case.num_cells
     forms = {"both": (64, 32), "mixed": (32,), "fp64": (64,)}[args.form]
@@ -206,6 +256,8 @@ def main():
                        "cells": nc, "report_steps": steps},
             "roofline": roof, "cpu_baseline": cpu}
     line.update(extra)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(args.adjoint_fd)
     print(json.dumps(line))
     return 0
 
