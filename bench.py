@@ -147,6 +147,61 @@ def adjoint_leg(fd_cells: int = 0, seed: int = 5):
     return out
 
 
+def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
+    """The uneven 256-member ensemble (5 x 30 d): the batched form (one
+    union system, FP32 CPR inside FP64 FGMRES) against the plain FP64 form
+    looped member by member over the first `loop_sample` members (a time
+    box; the per-member rate is what the metric compares)."""
+    import torch
+    from pymrst_amd.ensemble import build_ensemble
+    from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
+    from pymrst_amd.workloads import build_model, ensemble_cases
+    cases = ensemble_cases(members)
+    dts = cases[0].dts
+    solver = CPRSolver(precision=32)
+    t0 = time.perf_counter()
+    model, s0, ranges = build_ensemble(cases, solver)
+    build_s = time.perf_counter() - t0
+    if warmup:
+        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True)
+    model.reset()
+    solver.history.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    states, its = simulate_schedule_ad(s0, model, dts, keep_device=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ncells = model.nc
+    batched = {"value": round(ncells * int(its.sum()) / wall / 1e6, 3), "wall_s": round(wall, 3),
+               "newton_its": its.tolist(), "gmres_its": int(sum(h[0] for h in solver.history)),
+               "union_build_s": round(build_s, 1)}
+    del model, states
+    torch.cuda.empty_cache()
+    tot_cells_its, tot_wall, ncl = 0, 0.0, 0
+    for c in cases[:loop_sample]:
+        sv = CPRSolver(precision=64)
+        m, st0 = build_model(c, sv)
+        if warmup:
+            simulate_schedule_ad(st0, m, c.dts[:warmup], keep_device=True)
+        m.reset()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, its_m = simulate_schedule_ad(st0, m, c.dts, keep_device=True)
+        torch.cuda.synchronize()
+        tot_wall += time.perf_counter() - t1
+        tot_cells_its += c.num_cells * int(its_m.sum())
+        ncl += c.num_cells
+        del m
+    looped = {"members": loop_sample, "cells": ncl, "value": round(tot_cells_its / tot_wall / 1e6, 3),
+              "wall_s": round(tot_wall, 3)}
+    print(f"[bench] ensemble: batched {batched['value']} vs looped FP64 {looped['value']} Mcell-Newton-its/s",
+          file=sys.stderr, flush=True)
+    return {"workload": f"{members} oil-water realisations, nx,ny~U{{20..64}}, nz~U{{5..16}}, "
+                        f"sigma(ln k)~U(0.5,2), 5 x 30 d", "members": members, "cells": ncells,
+            "batched_fp32_precond": batched, "looped_plain_fp64_sample": looped,
+            "speedup_batched_vs_looped_fp64": round(batched["value"] / looped["value"], 2)}
+
+
 def traffic_probe(case, reps):
     """A short program for hardware-counter runs: the FP64 BCSR SpMV of the
     assembled Jacobian `reps` times, then a plain 512 MiB device copy `reps`
@@ -189,6 +244,8 @@ def main():
                     help="check the adjoint gradient by central differences on CELLS random cells")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
                     help="run only the SpMV and a calibration copy REPS times (for counter runs)")
     args = ap.parse_args()
@@ -205,6 +262,9 @@ def main():
     case = field_waterflood(dims=dims)
     if args.traffic_probe > 0:
         return traffic_probe(case, args.traffic_probe)
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg()}))
+        return 0
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_fd)}))
         return 0
@@ -258,6 +318,8 @@ def main():
     line.update(extra)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(args.adjoint_fd)
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_leg()
     print(json.dumps(line))
     return 0
 

@@ -46,14 +46,14 @@ STB = 42.0 * 231.0 * 0.0254 ** 3    # stock-tank barrel in m^3
 
 
 def field_waterflood(dims=(60, 220, 85), cell=(6.1, 3.05, 0.61), seed=1, steps=10, step_days=30.0,
-                     inj_rate=5000.0 * STB / DAY, prod_bhp=150 * BARSA) -> OWCase:
+                     inj_rate=5000.0 * STB / DAY, prod_bhp=150 * BARSA, sigma=1.5) -> OWCase:
     nx, ny, nz = dims
     nc = nx * ny * nz
     rng = np.random.default_rng(seed)
     medians_md = 10.0 ** rng.uniform(0.0, 3.0, nz)
     z = rng.standard_normal(nc)
     layer = np.arange(nc) // (nx * ny)
-    kx = np.exp(np.log(medians_md[layer]) + 1.5 * z) * MILLI * DARCY
+    kx = np.exp(np.log(medians_md[layer]) + sigma * z) * MILLI * DARCY
     perm = np.column_stack([kx, kx, 0.1 * kx])
     poro = 0.05 + 0.25 * ndtr(z)
     column = lambda i, j: i + nx * j + nx * ny * np.arange(nz)
@@ -111,3 +111,21 @@ def adjoint_problem(dims=(64, 64, 16), cell=(6.1, 3.05, 0.61), seed=4, steps=20,
     prob = AdjointProblem(g, np.full(nc, 0.2), fluid, specs, np.full(steps, step_days * DAY), 200 * BARSA, 0.2,
                           solver=solver, targets=targets, checkpoint_every=checkpoint_every)
     return prob, logk
+
+
+def ensemble_cases(members=256, seed=3, steps=5, step_days=30.0):
+    """The uneven ensemble: member m has nx, ny ~ U{20..64}, nz ~ U{5..16},
+    sigma(ln k) ~ U(0.5, 2.0) and its own seed, the field case's fluids,
+    cells and five-spot, with the injection rate scaled by the member's
+    areal extent against the 60x220 field (same rate per column of cells),
+    `steps` steps of `step_days` days."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for m in range(members):
+        nx, ny = (int(v) for v in rng.integers(20, 65, 2))
+        nz = int(rng.integers(5, 17))
+        sigma = float(rng.uniform(0.5, 2.0))
+        rate = 5000.0 * STB / DAY * (nx * ny) / (60 * 220)
+        cases.append(field_waterflood(dims=(nx, ny, nz), seed=int(rng.integers(1 << 31)), steps=steps,
+                                      step_days=step_days, inj_rate=rate, sigma=sigma))
+    return cases
