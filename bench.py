@@ -158,17 +158,24 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     from pymrst_amd.workloads import build_model, ensemble_cases
     cases = ensemble_cases(members)
     dts = cases[0].dts
-    solver = CPRSolver(precision=32)
+    solver = CPRSolver(precision=32, verbose=1)
     t0 = time.perf_counter()
     model, s0, ranges = build_ensemble(cases, solver)
     build_s = time.perf_counter() - t0
+    print(f"[bench] ensemble: union of {members} members, {model.nc} cells, {model.nw} wells, built in "
+          f"{build_s:.1f} s", file=sys.stderr, flush=True)
+
+    def progress(event, payload):
+        if event in ("step_accepted", "step_rejected", "timestep_cut"):
+            print(f"[bench] ensemble {event} {payload}", file=sys.stderr, flush=True)
+
     if warmup:
-        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True)
+        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True, observer=progress)
     model.reset()
     solver.history.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    states, its = simulate_schedule_ad(s0, model, dts, keep_device=True)
+    states, its = simulate_schedule_ad(s0, model, dts, keep_device=True, observer=progress)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ncells = model.nc
@@ -178,7 +185,8 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     del model, states
     torch.cuda.empty_cache()
     tot_cells_its, tot_wall, ncl = 0, 0.0, 0
-    for c in cases[:loop_sample]:
+    for k, c in enumerate(cases[:loop_sample]):
+        print(f"[bench] ensemble: looped member {k} ({c.num_cells} cells)", file=sys.stderr, flush=True)
         sv = CPRSolver(precision=64)
         m, st0 = build_model(c, sv)
         if warmup:
@@ -246,6 +254,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-loop", type=int, default=16, help="members timed in the looped plain form")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
                     help="run only the SpMV and a calibration copy REPS times (for counter runs)")
     args = ap.parse_args()
@@ -263,7 +273,7 @@ def main():
     if args.traffic_probe > 0:
         return traffic_probe(case, args.traffic_probe)
     if args.ensemble_only:
-        print(json.dumps({"ensemble": ensemble_leg()}))
+        print(json.dumps({"ensemble": ensemble_leg(args.ensemble_members, args.ensemble_loop)}))
         return 0
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_fd)}))
@@ -319,7 +329,7 @@ def main():
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(args.adjoint_fd)
     if not args.no_ensemble:
-        line["ensemble"] = ensemble_leg()
+        line["ensemble"] = ensemble_leg(args.ensemble_members, args.ensemble_loop)
     print(json.dumps(line))
     return 0
 

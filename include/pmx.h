@@ -134,6 +134,14 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4);
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);
 
+/* Independent members (an ensemble solved as one system): member s owns cells
+ * [cell_ptr[s], cell_ptr[s+1]) and wells [well_ptr[s], well_ptr[s+1]) (host
+ * arrays of nseg+1).  FGMRES then runs one Krylov recurrence per member
+ * (own dots, Hessenberg, Givens, convergence and restarts) while the
+ * preconditioner and the SpMV cover all members in each launch.  No
+ * reference counterpart: the reference loops over realisations. */
+int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const int32_t* well_ptr);
+
 /* ---- discrete adjoint (pymrst/modules/optimization/adjoint.py:73-225) */
 /* Per connection f (host, 2*nconn): T_f/T_h at N0 and at N1, the derivative
  * of the face transmissibility with respect to the log-permeability of each

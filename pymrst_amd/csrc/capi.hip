@@ -436,4 +436,12 @@ int pmx_ow_dlogk(pmx_ow* h, const double* p, const double* s, const double* qws,
   })
 }
 
+int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const int32_t* well_ptr) {
+  PMX_TRY({
+    PMX_CHECK(h && cell_ptr && well_ptr && nseg >= 1, "bad argument");
+    bind(h);
+    cpr_set_segments(h->m->cpr, nseg, cell_ptr, well_ptr, 2, 3, h->m->nc, h->m->nw);
+  })
+}
+
 }  // extern "C"

@@ -99,6 +99,10 @@ bool cpr_has_hierarchy(const CprSolver*);
 // Lines for the local stage (line_ptr[nlines+1], cells); without them lines
 // are detected from the strength weights.
 void cpr_set_lines(CprSolver*, int nlines, const int* ptr, const int* cells);
+// Independent segments (ensemble members): cells [cell_ptr[s], cell_ptr[s+1])
+// and wells [well_ptr[s], well_ptr[s+1]) form segment s; FGMRES then keeps
+// one Krylov recurrence per segment.
+void cpr_set_segments(CprSolver*, int nseg, const int* cell_ptr, const int* well_ptr, int B, int M, int nc, int nw);
 // Numeric setup for the current values of S (pressure weights: w_dev[B]).
 void cpr_setup(CprSolver*, const BlockSystem& S, const double* w_dev, const LinearSolverOptions& o,
                hipStream_t s);

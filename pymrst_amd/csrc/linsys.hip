@@ -1223,8 +1223,14 @@ struct AmgLevel {
   DArr<float> v32, d32, x32, y32, b32, r32;
 };
 
+struct SegFGMRES;
+int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, double* x,
+                        const LinearSolverOptions& o, SolveInfo& info, hipStream_t s);
+
 class CprSolver {
  public:
+  std::unique_ptr<SegFGMRES> seg;     // per-segment Krylov state (ensembles), when set
+  ~CprSolver();
   std::vector<std::unique_ptr<AmgLevel>> L;
   bool built = false;
   int coarse_n = 0;
@@ -1261,11 +1267,7 @@ class CprSolver {
   hipGraphExec_t gexec = nullptr;
   bool graph_ok = false;
   LinearSolverOptions graph_opts;
-  ~CprSolver() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (cap_s) (void)hipStreamDestroy(cap_s);
-    if (io) (void)hipFree(io);
-  }
+
   // repeated pressure cycles: original right-hand side and accumulated solution
   DArr<double> pb64, pacc64;
   DArr<float> pb32, pacc32;
@@ -1768,6 +1770,7 @@ void cpr_apply(CprSolver* C, const BlockSystem& S, const double* r, double* z, c
 // ---------------------------------------------------------------------------
 int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, const LinearSolverOptions& o,
               SolveInfo& info, hipStream_t s) {
+  if (C->seg) return cpr_solve_segmented(C, S, b, x, o, info, s);
   auto t0 = std::chrono::steady_clock::now();
   const size_t n = S.n();
   const int m = std::max(1, std::min(o.restart, o.maxiter));
@@ -1915,4 +1918,413 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   return status;
 }
 
+}  // namespace pmx
+
+// ---------------------------------------------------------------------------
+// Segmented FGMRES: independent right-hand sides solved together.
+//
+// The unknowns are split into segments (the members of an ensemble, each a
+// range of cell unknowns and a range of well unknowns).  The operator and
+// the preconditioner are block-diagonal over segments, so one launch of
+// each serves all of them, but every segment keeps its own Krylov
+// coefficients: its own Gram-Schmidt dots, Hessenberg matrix, Givens
+// rotations, convergence test (||b_s - J x_s|| <= rtol ||b_s||) and
+// restarts.  A single Krylov space over the union would have to damp every
+// segment's outlying eigenvalues with one polynomial, and stalls as the
+// segments multiply.  Reductions run over tiles that never straddle two
+// segments; a segment's sum is its tiles' partial sums in tile order, so
+// the result does not depend on timing.
+// ---------------------------------------------------------------------------
+namespace pmx {
+
+struct SegmentPlan {
+  int nseg = 0, ntile = 0;
+  DArr<int> tile_start, tile_len, tptr, tidx, seg_of;
+  std::vector<int> h_tptr;
+};
+
+__global__ __launch_bounds__(kRedThreads) void k_seg_multidot(int ntile, const int* __restrict__ tstart,
+                                                              const int* __restrict__ tlen, int nv,
+                                                              const double* __restrict__ V, size_t ld,
+                                                              const double* __restrict__ w,
+                                                              double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads / 64][kRedGroup];
+  const int tile = blockIdx.x;
+  if (tile >= ntile) return;
+  const int nd = nv + 1;
+  const size_t base = (size_t)tstart[tile];
+  const int len = tlen[tile];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double wv[kRedElems];
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    const int i = threadIdx.x + e * kRedThreads;
+    wv[e] = i < len ? w[base + i] : 0.0;
+  }
+  for (int d0 = 0; d0 < nd; d0 += kRedGroup) {
+    double acc[kRedGroup];
+#pragma unroll
+    for (int q = 0; q < kRedGroup; ++q) {
+      const int d = d0 + q;
+      double a = 0.0;
+      if (d < nv) {
+#pragma unroll
+        for (int e = 0; e < kRedElems; ++e) {
+          const int i = threadIdx.x + e * kRedThreads;
+          if (i < len) a = fma(V[(size_t)d * ld + base + i], wv[e], a);
+        }
+      } else if (d == nv) {
+#pragma unroll
+        for (int e = 0; e < kRedElems; ++e) a = fma(wv[e], wv[e], a);
+      }
+      acc[q] = wave_sum(a, 64);
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < kRedGroup; ++q) sh[wid][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < kRedGroup && d0 + (int)threadIdx.x < nd) {
+      double t = 0.0;
+#pragma unroll
+      for (int v = 0; v < kRedThreads / 64; ++v) t += sh[v][threadIdx.x];
+      partial[(size_t)tile * nd + d0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// out[s*nd + d] = sum over segment s's tiles (in order) of partial[tile*nd + d]
+__global__ void k_seg_reduce(int nseg, int nd, const int* __restrict__ tptr, const int* __restrict__ tidx,
+                             const double* __restrict__ partial, double* __restrict__ out) {
+  const int s = blockIdx.x;
+  for (int d = threadIdx.x; d < nd; d += blockDim.x) {
+    double t = 0.0;
+    for (int k = tptr[s]; k < tptr[s + 1]; ++k) t += partial[(size_t)tidx[k] * nd + d];
+    out[(size_t)s * nd + d] = t;
+  }
+}
+
+// w -= sum_d coef[seg(i)*ldc + d] V_d (per tile, so partial[tile] = |w|^2 over the tile)
+__global__ __launch_bounds__(kRedThreads) void k_seg_multiaxpy_norm(int ntile, const int* __restrict__ tstart,
+                                                                    const int* __restrict__ tlen,
+                                                                    const int* __restrict__ seg_of, int nv,
+                                                                    const double* __restrict__ V, size_t ld,
+                                                                    const double* __restrict__ coef, int ldc,
+                                                                    double* __restrict__ w,
+                                                                    double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads / 64];
+  const int tile = blockIdx.x;
+  if (tile >= ntile) return;
+  const size_t base = (size_t)tstart[tile];
+  const int len = tlen[tile];
+  const int sg = seg_of[base];
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    const int i = threadIdx.x + e * kRedThreads;
+    if (i < len) {
+      double wi = w[base + i];
+      for (int d = 0; d < nv; ++d) wi = fma(-coef[(size_t)sg * ldc + d], V[(size_t)d * ld + base + i], wi);
+      w[base + i] = wi;
+      acc = fma(wi, wi, acc);
+    }
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[tile] = s;
+}
+
+// w = (w - sum_d coef[seg,d] V_d) * scale[seg]
+__global__ void k_seg_multiaxpy_scale(size_t n, const int* __restrict__ seg_of, int nv, const double* __restrict__ V,
+                                      size_t ld, const double* __restrict__ coef, int ldc,
+                                      const double* __restrict__ scale, double* __restrict__ w) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sg = seg_of[i];
+  double wi = w[i];
+  for (int d = 0; d < nv; ++d) wi = fma(-coef[(size_t)sg * ldc + d], V[(size_t)d * ld + i], wi);
+  w[i] = wi * scale[sg];
+}
+
+// x += sum_d coef[seg,d] Z_d
+__global__ void k_seg_multiaxpy_add(size_t n, const int* __restrict__ seg_of, int nv, const double* __restrict__ Z,
+                                    size_t ld, const double* __restrict__ coef, int ldc, double* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sg = seg_of[i];
+  double xi = x[i];
+  for (int d = 0; d < nv; ++d) xi = fma(coef[(size_t)sg * ldc + d], Z[(size_t)d * ld + i], xi);
+  x[i] = xi;
+}
+
+__global__ void k_seg_scale_copy(size_t n, const int* __restrict__ seg_of, const double* __restrict__ scale,
+                                 const double* __restrict__ x, double* __restrict__ y) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = scale[seg_of[i]] * x[i];
+}
+
+struct SegFGMRES {
+  SegmentPlan plan;
+  DArr<double> partial, dots, coef, scale;
+};
+
+void cpr_set_segments(CprSolver* C, int nseg, const int* cell_ptr, const int* well_ptr, int B, int M, int nc,
+                      int nw) {
+  PMX_CHECK(nseg >= 1, "need at least one segment");
+  PMX_CHECK(cell_ptr[0] == 0 && cell_ptr[nseg] == nc, "segment cell ranges must cover the cells");
+  PMX_CHECK(well_ptr[0] == 0 && well_ptr[nseg] == nw, "segment well ranges must cover the wells");
+  if (!C->seg) C->seg = std::make_unique<SegFGMRES>();
+  SegmentPlan& P = C->seg->plan;
+  const size_t n = (size_t)nc * B + (size_t)nw * M;
+  std::vector<int> ts, tl, tsg, seg_of(n);
+  auto add_range = [&](size_t a, size_t b, int s) {
+    for (size_t i = a; i < b; ++i) seg_of[i] = s;
+    for (size_t t = a; t < b; t += kRedTile) {
+      ts.push_back((int)t);
+      tl.push_back((int)std::min<size_t>(kRedTile, b - t));
+      tsg.push_back(s);
+    }
+  };
+  for (int s = 0; s < nseg; ++s) {
+    PMX_CHECK(cell_ptr[s + 1] >= cell_ptr[s] && well_ptr[s + 1] >= well_ptr[s], "bad segment ranges");
+    add_range((size_t)cell_ptr[s] * B, (size_t)cell_ptr[s + 1] * B, s);
+    add_range((size_t)nc * B + (size_t)well_ptr[s] * M, (size_t)nc * B + (size_t)well_ptr[s + 1] * M, s);
+  }
+  const int nt = (int)ts.size();
+  std::vector<int> tptr(nseg + 1, 0), tidx(nt);
+  for (int t = 0; t < nt; ++t) tptr[tsg[t] + 1]++;
+  for (int s = 0; s < nseg; ++s) tptr[s + 1] += tptr[s];
+  std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+  for (int t = 0; t < nt; ++t) tidx[fill[tsg[t]]++] = t;
+  hipStream_t s0 = nullptr;
+  P.nseg = nseg;
+  P.ntile = nt;
+  P.tile_start.upload(ts, s0);
+  P.tile_len.upload(tl, s0);
+  P.tptr.upload(tptr, s0);
+  P.tidx.upload(tidx, s0);
+  P.seg_of.upload(seg_of, s0);
+  P.h_tptr = tptr;
+  PMX_HIP(hipStreamSynchronize(s0));
+}
+
+int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, double* x,
+                        const LinearSolverOptions& o, SolveInfo& info, hipStream_t s) {
+  auto t0 = std::chrono::steady_clock::now();
+  SegFGMRES& F = *C->seg;
+  SegmentPlan& P = F.plan;
+  const int ns = P.nseg;
+  const size_t n = S.n();
+  PMX_CHECK(P.seg_of.n == n, "segments do not match the system");
+  const int m = std::max(1, std::min(o.restart, o.maxiter));
+  const size_t ld = (n + 63) / 64 * 64;
+  if (C->m_alloc < m || C->n_alloc != n) {
+    C->V.alloc(ld * (m + 1));
+    C->Z.alloc(ld * m);
+    C->r.alloc(ld);
+    C->hmat.alloc((size_t)std::max(S.nw, 1) * kMaxMix);
+    C->m_alloc = m;
+    C->n_alloc = n;
+    C->ld = ld;
+  }
+  F.partial.alloc((size_t)P.ntile * (m + 2));
+  F.dots.alloc((size_t)ns * (m + 2));
+  F.coef.alloc((size_t)ns * (m + 1));
+  F.scale.alloc(ns);
+  double* hscr = C->hmat.p;
+  const int nb = grid_for(n);
+  auto seg_dots = [&](int nv, const double* w, std::vector<double>& out) {
+    const int nd = nv + 1;
+    hipLaunchKernelGGL(k_seg_multidot, dim3(P.ntile), dim3(kRedThreads), 0, s, P.ntile, P.tile_start.p,
+                       P.tile_len.p, nv, C->V.p, ld, w, F.partial.p);
+    launch_check();
+    hipLaunchKernelGGL(k_seg_reduce, dim3(ns), dim3(64), 0, s, ns, nd, P.tptr.p, P.tidx.p, F.partial.p, F.dots.p);
+    launch_check();
+    out.resize((size_t)ns * nd);
+    PMX_HIP(hipMemcpyAsync(out.data(), F.dots.p, sizeof(double) * ns * nd, hipMemcpyDeviceToHost, s));
+    PMX_HIP(hipStreamSynchronize(s));
+  };
+  // ||b_s|| (V holds nothing yet: nv = 0 gives only w.w)
+  std::vector<double> dd;
+  seg_dots(0, b, dd);
+  std::vector<double> bnorm(ns), tol(ns), beta(ns);
+  for (int k = 0; k < ns; ++k) {
+    bnorm[k] = std::sqrt(std::max(0.0, dd[k]));
+    tol[k] = o.rtol * bnorm[k];
+  }
+  PMX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+  info = SolveInfo();
+  std::vector<char> fin(ns, 0);   // converged for good
+  for (int k = 0; k < ns; ++k) {
+    if (!std::isfinite(bnorm[k])) return -1;
+    fin[k] = !(bnorm[k] > 0.0);
+    beta[k] = bnorm[k];
+  }
+  PMX_HIP(hipMemcpyAsync(C->r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  std::vector<double> H((size_t)ns * (m + 1) * m), cs((size_t)ns * m), sn((size_t)ns * m), g((size_t)ns * (m + 1)),
+      y((size_t)ns * m), hscale(ns), hcoef((size_t)ns * (m + 1));
+  std::vector<int> kdone(ns);
+  int total = 0, status = 1, cycles = 0;
+  auto all_fin = [&]() {
+    for (int k = 0; k < ns; ++k)
+      if (!fin[k]) return false;
+    return true;
+  };
+  double worst = 0.0;
+  while (cycles < o.maxiter && !all_fin()) {
+    ++cycles;
+    for (int k = 0; k < ns; ++k) hscale[k] = fin[k] ? 0.0 : 1.0 / beta[k];
+    PMX_HIP(hipMemcpyAsync(F.scale.p, hscale.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_seg_scale_copy, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, F.scale.p, C->r.p, C->V.p);
+    launch_check();
+    std::fill(g.begin(), g.end(), 0.0);
+    std::vector<char> active(ns);
+    for (int k = 0; k < ns; ++k) {
+      active[k] = !fin[k];
+      g[(size_t)k * (m + 1)] = beta[k];
+      kdone[k] = 0;
+    }
+    int j = 0;
+    for (; j < m; ++j) {
+      double* vj = C->V.p + (size_t)j * ld;
+      double* zj = C->Z.p + (size_t)j * ld;
+      double* w = C->V.p + (size_t)(j + 1) * ld;
+      C->prec_t.start(s);
+      cpr_apply(C, S, vj, zj, o, s);
+      C->prec_t.stop(s);
+      system_matvec(S, zj, w, hscr, s, &C->spmv_t);
+      const int nd = j + 2;
+      std::vector<double> hh;
+      seg_dots(j + 1, w, hh);
+      std::vector<double> hcol((size_t)ns * (j + 2)), wn2(ns);
+      bool reorth = false;
+      for (int k = 0; k < ns; ++k) {
+        double hsq = 0.0;
+        for (int i = 0; i <= j; ++i) {
+          const double h = active[k] ? hh[(size_t)k * nd + i] : 0.0;
+          hcol[(size_t)k * (j + 2) + i] = h;
+          hsq += h * h;
+        }
+        wn2[k] = hh[(size_t)k * nd + j + 1] - hsq;
+        if (active[k] && o.reorthogonalize && !(wn2[k] > 0.5 * hh[(size_t)k * nd + j + 1])) reorth = true;
+      }
+      // coefficients of the first projection (zero for inactive segments)
+      for (int k = 0; k < ns; ++k)
+        for (int i = 0; i <= j; ++i) hcoef[(size_t)k * (m + 1) + i] = hcol[(size_t)k * (j + 2) + i];
+      PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * (m + 1), hipMemcpyHostToDevice, s));
+      if (reorth) {
+        hipLaunchKernelGGL(k_seg_multiaxpy_norm, dim3(P.ntile), dim3(kRedThreads), 0, s, P.ntile, P.tile_start.p,
+                           P.tile_len.p, P.seg_of.p, j + 1, C->V.p, ld, F.coef.p, m + 1, w, F.partial.p);
+        launch_check();
+        std::vector<double> h2;
+        seg_dots(j + 1, w, h2);
+        for (int k = 0; k < ns; ++k) {
+          double h2sq = 0.0;
+          for (int i = 0; i <= j; ++i) {
+            const double h = active[k] ? h2[(size_t)k * nd + i] : 0.0;
+            hcol[(size_t)k * (j + 2) + i] += h;
+            hcoef[(size_t)k * (m + 1) + i] = h;
+            h2sq += h * h;
+          }
+          wn2[k] = h2[(size_t)k * nd + j + 1] - h2sq;
+        }
+        PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * (m + 1), hipMemcpyHostToDevice, s));
+        ++info.reorthogonalizations;
+      }
+      ++total;
+      bool any_active = false;
+      for (int k = 0; k < ns; ++k) {
+        hscale[k] = 0.0;
+        if (!active[k]) continue;
+        double* hc = &hcol[(size_t)k * (j + 2)];
+        const double hnext = std::sqrt(std::max(wn2[k], 0.0));
+        hc[j + 1] = hnext;
+        if (!std::isfinite(hnext)) return -1;
+        double* csk = &cs[(size_t)k * m];
+        double* snk = &sn[(size_t)k * m];
+        double* gk = &g[(size_t)k * (m + 1)];
+        for (int i = 0; i < j; ++i) {
+          const double t = csk[i] * hc[i] + snk[i] * hc[i + 1];
+          hc[i + 1] = -snk[i] * hc[i] + csk[i] * hc[i + 1];
+          hc[i] = t;
+        }
+        const double a = hc[j], bb = hc[j + 1];
+        const double rr = std::hypot(a, bb);
+        csk[j] = rr > 0 ? a / rr : 1.0;
+        snk[j] = rr > 0 ? bb / rr : 0.0;
+        hc[j] = rr;
+        gk[j + 1] = -snk[j] * gk[j];
+        gk[j] = csk[j] * gk[j];
+        for (int i = 0; i <= j; ++i) H[(size_t)k * (m + 1) * m + (size_t)i * m + j] = hc[i];
+        kdone[k] = j + 1;
+        if (std::fabs(gk[j + 1]) <= tol[k] || !(hnext > 0.0)) {
+          active[k] = 0;   // this segment's cycle ends here
+        } else {
+          hscale[k] = 1.0 / hnext;
+          any_active = true;
+        }
+      }
+      if (!any_active) {
+        ++j;
+        break;
+      }
+      // V_{j+1} = (w - V h) / ||w - V h|| per segment (zero for the finished)
+      PMX_HIP(hipMemcpyAsync(F.scale.p, hscale.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_seg_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, j + 1, C->V.p, ld,
+                         F.coef.p, m + 1, F.scale.p, w);
+      launch_check();
+    }
+    // y_s = R_s^{-1} g_s over each segment's own iterations
+    std::fill(hcoef.begin(), hcoef.end(), 0.0);
+    for (int k = 0; k < ns; ++k) {
+      const int jk = kdone[k];
+      const double* Hk = &H[(size_t)k * (m + 1) * m];
+      for (int i = jk - 1; i >= 0; --i) {
+        double t = g[(size_t)k * (m + 1) + i];
+        for (int q = i + 1; q < jk; ++q) t -= Hk[(size_t)i * m + q] * hcoef[(size_t)k * (m + 1) + q];
+        hcoef[(size_t)k * (m + 1) + i] = t / Hk[(size_t)i * m + i];
+      }
+    }
+    PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * (m + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_seg_multiaxpy_add, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, j, C->Z.p, ld, F.coef.p,
+                       m + 1, x);
+    launch_check();
+    // true residuals
+    system_matvec(S, x, C->r.p, hscr, s);
+    hipLaunchKernelGGL(k_axpby, dim3(nb), dim3(256), 0, s, n, 1.0, b, -1.0, C->r.p);
+    launch_check();
+    std::vector<double> rr;
+    seg_dots(0, C->r.p, rr);
+    worst = 0.0;
+    for (int k = 0; k < ns; ++k) {
+      beta[k] = std::sqrt(std::max(0.0, rr[k]));
+      if (!std::isfinite(beta[k])) return -1;
+      if (beta[k] <= tol[k]) fin[k] = 1;
+      if (bnorm[k] > 0.0) worst = std::max(worst, beta[k] / bnorm[k]);
+    }
+    if (all_fin()) {
+      status = 0;
+      break;
+    }
+    ++info.restarts;
+  }
+  info.iterations = total;
+  info.rel_residual = worst;
+  info.levels = (int)C->L.size();
+  info.coarse_n = C->coarse_n;
+  info.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (o.verbose)
+    fprintf(stderr, "[pmx] segmented fgmres (%d segments, FP%d preconditioner): %d its (%d restarts), worst rel "
+                    "residual %.2e, %.2f ms, status %d\n",
+            ns, C->prec_setup, total, info.restarts, worst, info.solve_ms, status);
+  return status;
+}
+
+}  // namespace pmx
+
+namespace pmx {
+CprSolver::~CprSolver() {
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (cap_s) (void)hipStreamDestroy(cap_s);
+  if (io) (void)hipFree(io);
+}
 }  // namespace pmx

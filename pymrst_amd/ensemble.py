@@ -2,8 +2,11 @@
 
 The members' grids are joined into one disjoint union (cells and
 connections offset, wells renumbered), so one assembly launch, one CPR setup
-and one FGMRES cover every member, and the AMG hierarchy and line tables
-simply have one component per member.  Newton runs on the union: an
+and one preconditioner or SpMV launch cover every member, and the AMG
+hierarchy and line tables simply have one component per member.  FGMRES
+keeps one Krylov recurrence per member (pmx_ow_set_segments): a single
+Krylov space over the union stalls as members multiply, because one
+polynomial has to damp every member's outlying eigenvalues.  Newton runs on the union: an
 iteration is taken while any member is unconverged, and convergence is
 judged member by member with the reference's CNV / MB / well tests
 (two_phase_oil_water_model.py _converged :549-567), so a member is never
@@ -87,9 +90,11 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
         return bool(ok_m.all()), float(p.mean())
 
 
-def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, device=None) -> tuple:
+def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, device=None,
+                   segmented: bool = True) -> tuple:
     """(EnsembleModel, initial host state, member cell ranges) for a list of
-    OWCase members sharing one fluid (the first member's)."""
+    OWCase members sharing one fluid (the first member's).  With
+    `segmented`, FGMRES keeps one Krylov recurrence per member."""
     c0 = cases[0]
     fluid = init_simple_adi_fluid(c0.mu, c0.rho, c0.n, c0.c, c0.p_ref, c_r=c0.c_r)
     Ns, Ts, cents, vols, perms, poros, wells, lines_ptr, lines_cells = [], [], [], [], [], [], [], [0], []
@@ -128,4 +133,10 @@ def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, d
     _lib.check(model.lib.pmx_ow_set_lines(model.h, ptr.size - 1, _lib.ptr(ptr), _lib.ptr(cells)),
                "pmx_ow_set_lines")
     model.setup_members(np.concatenate(cell_member), np.asarray(well_member), len(cases))
+    if segmented:
+        cptr = np.ascontiguousarray([0] + [b for _, b in ranges], dtype=np.int32)
+        wptr = np.ascontiguousarray(np.concatenate([[0], np.cumsum(np.bincount(well_member, minlength=len(cases)))]),
+                                    dtype=np.int32)
+        _lib.check(model.lib.pmx_ow_set_segments(model.h, len(cases), _lib.ptr(cptr), _lib.ptr(wptr)),
+                   "pmx_ow_set_segments")
     return model, model.initial_state(c0.p0, c0.sw0), ranges

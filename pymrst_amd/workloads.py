@@ -117,15 +117,16 @@ def ensemble_cases(members=256, seed=3, steps=5, step_days=30.0):
     """The uneven ensemble: member m has nx, ny ~ U{20..64}, nz ~ U{5..16},
     sigma(ln k) ~ U(0.5, 2.0) and its own seed, the field case's fluids,
     cells and five-spot, with the injection rate scaled by the member's
-    areal extent against the 60x220 field (same rate per column of cells),
-    `steps` steps of `step_days` days."""
+    cell count against the 60x220x85 field (the same injection per cell, so
+    every member sees the field case's flood speed), `steps` steps of
+    `step_days` days."""
     rng = np.random.default_rng(seed)
     cases = []
     for m in range(members):
         nx, ny = (int(v) for v in rng.integers(20, 65, 2))
         nz = int(rng.integers(5, 17))
         sigma = float(rng.uniform(0.5, 2.0))
-        rate = 5000.0 * STB / DAY * (nx * ny) / (60 * 220)
+        rate = 5000.0 * STB / DAY * (nx * ny * nz) / (60 * 220 * 85)
         cases.append(field_waterflood(dims=(nx, ny, nz), seed=int(rng.integers(1 << 31)), steps=steps,
                                       step_days=step_days, inj_rate=rate, sigma=sigma))
     return cases
