@@ -141,6 +141,10 @@ int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r,
  * preconditioner and the SpMV cover all members in each launch.  No
  * reference counterpart: the reference loops over realisations. */
 int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const int32_t* well_ptr);
+/* Multiply the assembled residual by mask (device, one value per unknown):
+ * zero rows of members that have converged, so their Newton update is zero
+ * while the others iterate.  No reference counterpart. */
+int pmx_ow_mask_residual(pmx_ow* h, const double* mask);
 
 /* ---- discrete adjoint (pymrst/modules/optimization/adjoint.py:73-225) */
 /* Per connection f (host, 2*nconn): T_f/T_h at N0 and at N1, the derivative

@@ -444,4 +444,12 @@ int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const 
   })
 }
 
+int pmx_ow_mask_residual(pmx_ow* h, const double* mask) {
+  PMX_TRY({
+    PMX_CHECK(h && mask, "null argument");
+    bind(h);
+    h->m->mask_residual(mask);
+  })
+}
+
 }  // extern "C"

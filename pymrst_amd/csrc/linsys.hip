@@ -1768,6 +1768,17 @@ void cpr_apply(CprSolver* C, const BlockSystem& S, const double* r, double* z, c
 // Gram-Schmidt (optionally twice).  Converged when ||b - J x|| <= rtol ||b||,
 // the outer test of the reference's scipy gmres call.
 // ---------------------------------------------------------------------------
+// Repeated line Gauss-Seidel sweeps are not a contraction on every system:
+// when a restart cycle leaves the residual above half of what it started
+// from, the rest of the solve uses the cell block-Jacobi local stage
+// (flexible GMRES allows the preconditioner to change between iterations).
+static bool stagnated(LinearSolverOptions& lo, double now, double before, int verbose) {
+  if (lo.local_stage != 1 || !(now > 0.5 * before)) return false;
+  lo.local_stage = 0;
+  if (verbose) fprintf(stderr, "[pmx] local stage: line Gauss-Seidel stalled, block-Jacobi for the rest of the solve\n");
+  return true;
+}
+
 int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, const LinearSolverOptions& o,
               SolveInfo& info, hipStream_t s) {
   if (C->seg) return cpr_solve_segmented(C, S, b, x, o, info, s);
@@ -1801,7 +1812,8 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   }
   const double tol = o.rtol * bnorm;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hcol(m + 2), y(m);
-  double beta = bnorm;
+  double beta = bnorm, beta_cycle = bnorm;
+  LinearSolverOptions lo = o;   // the local stage may fall back below
   // r = b
   PMX_HIP(hipMemcpyAsync(C->r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   int total = 0;
@@ -1823,7 +1835,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       double* zj = C->Z.p + (size_t)j * ld;
       double* w = C->V.p + (size_t)(j + 1) * ld;
       C->prec_t.start(s);
-      cpr_apply(C, S, vj, zj, o, s);
+      cpr_apply(C, S, vj, zj, lo, s);
       C->prec_t.stop(s);
       system_matvec(S, zj, w, hscr, s, &C->spmv_t);
       // Classical Gram-Schmidt: h = V^T w and w.w in one pass; the norm of
@@ -1907,6 +1919,8 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     if (!std::isfinite(beta)) { status = -1; break; }
     if (beta <= tol) { status = 0; break; }
     ++info.restarts;
+    stagnated(lo, beta, beta_cycle, o.verbose);
+    beta_cycle = beta;
   }
   info.iterations = total;
   info.levels = (int)C->L.size();
@@ -2169,7 +2183,9 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       if (!fin[k]) return false;
     return true;
   };
-  double worst = 0.0;
+  double worst = 0.0, worst_cycle = 1.0;
+  int worst_seg = -1;
+  LinearSolverOptions lo = o;   // the local stage may fall back below
   while (cycles < o.maxiter && !all_fin()) {
     ++cycles;
     for (int k = 0; k < ns; ++k) hscale[k] = fin[k] ? 0.0 : 1.0 / beta[k];
@@ -2189,7 +2205,7 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       double* zj = C->Z.p + (size_t)j * ld;
       double* w = C->V.p + (size_t)(j + 1) * ld;
       C->prec_t.start(s);
-      cpr_apply(C, S, vj, zj, o, s);
+      cpr_apply(C, S, vj, zj, lo, s);
       C->prec_t.stop(s);
       system_matvec(S, zj, w, hscr, s, &C->spmv_t);
       const int nd = j + 2;
@@ -2299,13 +2315,18 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       beta[k] = std::sqrt(std::max(0.0, rr[k]));
       if (!std::isfinite(beta[k])) return -1;
       if (beta[k] <= tol[k]) fin[k] = 1;
-      if (bnorm[k] > 0.0) worst = std::max(worst, beta[k] / bnorm[k]);
+      if (bnorm[k] > 0.0 && beta[k] / bnorm[k] > worst) {
+        worst = beta[k] / bnorm[k];
+        worst_seg = k;
+      }
     }
     if (all_fin()) {
       status = 0;
       break;
     }
     ++info.restarts;
+    stagnated(lo, worst, worst_cycle, o.verbose);
+    worst_cycle = worst;
   }
   info.iterations = total;
   info.rel_residual = worst;
@@ -2314,8 +2335,9 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
   info.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (o.verbose)
     fprintf(stderr, "[pmx] segmented fgmres (%d segments, FP%d preconditioner): %d its (%d restarts), worst rel "
-                    "residual %.2e, %.2f ms, status %d\n",
-            ns, C->prec_setup, total, info.restarts, worst, info.solve_ms, status);
+                    "residual %.2e (segment %d, |b| %.3e), %.2f ms, status %d\n",
+            ns, C->prec_setup, total, info.restarts, worst, worst_seg, worst_seg >= 0 ? bnorm[worst_seg] : 0.0,
+            info.solve_ms, status);
   return status;
 }
 

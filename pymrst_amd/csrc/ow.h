@@ -25,6 +25,7 @@ class OWModelDev {
   int solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean);
   void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
               double* qos, double* bhp);
+  void mask_residual(const double* mask);   // r *= mask (device, per unknown)
 
   // adjoint (adjoint.hip): transposed solve of the last assembled Jacobian,
   // the previous-state coupling and the log-permeability derivative

@@ -492,6 +492,17 @@ int OWModelDev::solve(const LinearSolverOptions& o, double* dx, SolveInfo& info,
   return st;
 }
 
+__global__ void k_mask(size_t n, const double* __restrict__ mask, double* __restrict__ r) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] *= mask[i];
+}
+
+void OWModelDev::mask_residual(const double* mask) {
+  PMX_CHECK(r.n >= S.n(), "assemble before masking the residual");
+  hipLaunchKernelGGL(k_mask, dim3(grid_for(S.n())), dim3(256), 0, stream, S.n(), mask, r.p);
+  PMX_HIP(hipGetLastError());
+}
+
 void OWModelDev::update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s,
                         double* qws, double* qos, double* bhp) {
   hipLaunchKernelGGL(k_ow_update, dim3(grid_for((size_t)nc + nw)), dim3(256), 0, stream, nc, nw, dx, sign, ds_max,

@@ -56,6 +56,11 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
         qref = np.where(has_rate, np.maximum(qref, 1e-12), 1.0)
         self.qref_member = torch.as_tensor(qref, device=dev)
         self.well_is_bhp = torch.as_tensor([w.type == "bhp" for w in self.wells], device=dev)
+        # unknown -> member, for freezing converged members
+        self.unknown_member = torch.cat([self.cell_member.repeat_interleave(2),
+                                         self.well_member.repeat_interleave(3)])
+        self.member_its = np.zeros(nmem, dtype=np.int64)
+        self._ok_m = None
 
     def _seg(self, v, op, idx, n):
         out = torch.zeros(n, dtype=v.dtype, device=v.device)
@@ -87,7 +92,23 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
             ok_m &= bad == 0
         self.unconverged_members = int((~ok_m).sum())
         self.last_cnv_mb = (float(cnv.max()), float(mb.max()))
+        self._ok_m = ok_m
         return bool(ok_m.all()), float(p.mean())
+
+    def solve_and_update(self, st, pmean: float) -> bool:
+        """Members that have converged keep their state: their residual rows
+        are zeroed, so FGMRES gives them a zero update (as if each member ran
+        its own Newton loop), and only the others count an iteration."""
+        if self._ok_m is not None:
+            mask = (~self._ok_m)[self.unknown_member].to(torch.float64)
+            _lib.check(self.lib.pmx_ow_mask_residual(self.h, _lib.ptr(mask)), "pmx_ow_mask_residual")
+            self.member_its += (~self._ok_m).cpu().numpy()
+        return super().solve_and_update(st, pmean)
+
+    def reset(self):
+        super().reset()
+        if hasattr(self, "member_its"):
+            self.member_its[:] = 0
 
 
 def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, device=None,
@@ -139,4 +160,8 @@ def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, d
                                     dtype=np.int32)
         _lib.check(model.lib.pmx_ow_set_segments(model.h, len(cases), _lib.ptr(cptr), _lib.ptr(wptr)),
                    "pmx_ow_set_segments")
-    return model, model.initial_state(c0.p0, c0.sw0), ranges
+    st = model.initial_state(c0.p0, c0.sw0)
+    for c, (a, b) in zip(cases, ranges):
+        if c.p_init is not None:
+            st.pressure[a:b] = c.p_init
+    return model, st, ranges

@@ -12,6 +12,7 @@ pressure 200 bar and water saturation 0.2, gravity on, 10 steps of 30 days.  Eve
 can be rebuilt anywhere.
 """
 from dataclasses import dataclass
+from typing import Optional
 
 import numpy as np
 from scipy.special import ndtr
@@ -36,6 +37,7 @@ class OWCase:
     sw0: float
     dts: np.ndarray
     gravity: float
+    p_init: Optional[np.ndarray] = None   # per-cell initial pressure (None: p0 everywhere)
 
     @property
     def num_cells(self):
@@ -80,7 +82,10 @@ def build_model(case: OWCase, linear_solver=None, device=None):
     fluid = ow.init_simple_adi_fluid(case.mu, case.rho, case.n, case.c, case.p_ref, c_r=case.c_r)
     model = ow.TwoPhaseOilWaterModelAD(g, rock, fluid, wells, gravity=case.gravity,
                                        linear_solver=linear_solver, device=device)
-    return model, model.initial_state(case.p0, case.sw0)
+    st = model.initial_state(case.p0, case.sw0)
+    if case.p_init is not None:
+        st.pressure = np.asarray(case.p_init, float).copy()
+    return model, st
 
 
 def adjoint_problem(dims=(64, 64, 16), cell=(6.1, 3.05, 0.61), seed=4, steps=20, step_days=15.0,
@@ -113,20 +118,27 @@ def adjoint_problem(dims=(64, 64, 16), cell=(6.1, 3.05, 0.61), seed=4, steps=20,
     return prob, logk
 
 
-def ensemble_cases(members=256, seed=3, steps=5, step_days=30.0):
+def ensemble_cases(members=256, seed=3, steps=5, step_days=30.0, rate_scale=0.25):
     """The uneven ensemble: member m has nx, ny ~ U{20..64}, nz ~ U{5..16},
     sigma(ln k) ~ U(0.5, 2.0) and its own seed, the field case's fluids,
     cells and five-spot, with the injection rate scaled by the member's
-    cell count against the 60x220x85 field (the same injection per cell, so
-    every member sees the field case's flood speed), `steps` steps of
-    `step_days` days."""
+    cell count against the 60x220x85 field times `rate_scale` (0.25: a
+    quarter of the field's injection per cell, which every member alone
+    takes in 30-day steps without cuts), `steps` steps of `step_days`
+    days."""
     rng = np.random.default_rng(seed)
+    cell_dz = 0.61
     cases = []
     for m in range(members):
         nx, ny = (int(v) for v in rng.integers(20, 65, 2))
         nz = int(rng.integers(5, 17))
         sigma = float(rng.uniform(0.5, 2.0))
-        rate = 5000.0 * STB / DAY * (nx * ny * nz) / (60 * 220 * 85)
-        cases.append(field_waterflood(dims=(nx, ny, nz), seed=int(rng.integers(1 << 31)), steps=steps,
-                                      step_days=step_days, inj_rate=rate, sigma=sigma))
+        rate = rate_scale * 5000.0 * STB / DAY * (nx * ny * nz) / (60 * 220 * 85)
+        c = field_waterflood(dims=(nx, ny, nz), seed=int(rng.integers(1 << 31)), steps=steps,
+                             step_days=step_days, inj_rate=rate, sigma=sigma)
+        # start in hydrostatic equilibrium of the initial fluid column
+        rho0 = c.sw0 * c.rho[0] + (1.0 - c.sw0) * c.rho[1]
+        z = (np.arange(c.num_cells) // (nx * ny) + 0.5) * cell_dz
+        c.p_init = c.p0 + c.gravity * rho0 * (z - 0.5 * cell_dz)
+        cases.append(c)
     return cases
