@@ -168,6 +168,69 @@ int pmx_ow_prev_state_transpose(pmx_ow* h, const double* p0, const double* s0, d
 int pmx_ow_dlogk(pmx_ow* h, const double* p, const double* s, const double* qws, const double* qos,
                  const double* bhp, const double* cdp, const double* lam, double* grad);
 
+/* ---- three-phase live-oil black-oil model (ThreePhaseBlackOilModelAD,
+ *      pymrst/modules/ad_blackoil/models/three_phase_black_oil_model.py).
+ * Device unknowns: cell-major (p, sW, x) with x = rs where status == 1 and
+ * x = sG where status == 3, then well-major (qWs, qOs, qGs, bhp); residual
+ * rows (water, oil, gas) per cell and (eq_w, eq_o, eq_g, ctrl) per well;
+ * 3x3 row-major Jacobian blocks.  State arrays are device memory: p[nc],
+ * s[3*nc] (sW, sO, sG per cell), rs[nc], status[nc] (int32: 1 or 3),
+ * q[3*nw] (qWs, qOs, qGs per well), bhp[nw]. */
+typedef struct pmx_bo pmx_bo;
+
+/* Create the model with the deck fluid (init_deck_blackoil_fluid,
+ * pymrst/modules/ad_props/props/init_deck_adi_fluid.py:63-194), host arrays:
+ * scalars[9] = {rhoS_w, rhoS_o, rhoS_g (DENSITY), PVTW p_ref, B_ref, c, mu,
+ * ROCK p_ref, c}; pvdg[3*n_pvdg] rows (p, Bg, mu_g); PVTO as n_key Rs keys
+ * with rows branch[3*i] = (p, Bo, mu_o) for i in [branch_ptr[k],
+ * branch_ptr[k+1]) (first row saturated); swof[4*n_swof] and sgof[4*n_sgof]
+ * rows (s, kr, kro, pc). */
+int pmx_bo_create(int32_t device, void* hip_stream, const double* scalars, int32_t n_pvdg, const double* pvdg,
+                  int32_t n_key, const double* keys, const int32_t* branch_ptr, const double* branch,
+                  int32_t n_swof, const double* swof, int32_t n_sgof, const double* sgof, pmx_bo** out);
+int pmx_bo_destroy(pmx_bo* h);
+/* Connections, transmissibilities, gravity head g*dz per connection and
+ * reference pore volumes (host), as the model's constructor builds them
+ * (three_phase_black_oil_model.py:116-160). */
+int pmx_bo_set_topology(pmx_bo* h, int32_t nc, int32_t nconn, const int32_t* N, const double* T, const double* gdz,
+                        const double* pv);
+/* Wells (host): perforations grouped by well, WI per perforation, type 0 bhp /
+ * 1 total surface rate, target, compi[3*nw] (water, oil, gas) injection mix. */
+int pmx_bo_set_wells(pmx_bo* h, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells, const double* WI,
+                     const int32_t* type, const double* val, const double* compi);
+/* Lines for the line local stage of CPR (as pmx_ow_set_lines). */
+int pmx_bo_set_lines(pmx_bo* h, int32_t nlines, const int32_t* line_ptr, const int32_t* cells);
+/* Record the start-of-step accumulation terms of state0 (device). */
+int pmx_bo_begin_step(pmx_bo* h, const double* p0, const double* s0, const double* rs0, const int32_t* status0);
+/* Residual and Jacobian at the state (assemble :314-381 with _well
+ * :383-436); device arrays. */
+int pmx_bo_assemble(pmx_bo* h, const double* p, const double* s, const double* rs, const int32_t* status,
+                    const double* q, const double* bhp, double dt);
+/* Copy the last residual into out (device, 3*nc + 4*nw). */
+int pmx_bo_residual(pmx_bo* h, double* out);
+/* Reductions for _converged (:512-545), host out12 = {max|r_a/pv(p)| (3),
+ * sum r_a (3), sum 1/b_a (3), sum p, sum rs, sum pv(p)}; wells_out (host,
+ * 4*nw) gets the well rows.  p, rs, status are the device state. */
+int pmx_bo_convergence(pmx_bo* h, const double* p, const double* rs, const int32_t* status, double* out12,
+                       double* wells_out);
+/* Newton increment J dx = -r (step :547-571, a direct solve there) by
+ * CPR-FGMRES with pressure weights at (pmean, rsmean). */
+int pmx_bo_solve(pmx_bo* h, const pmx_solver_options* o, double pmean, double rsmean, double* dx,
+                 pmx_solve_info* info);
+/* State update with the variable switch and the flash (update_state
+ * :439-470, _flash :472-509), in place on the device state. */
+int pmx_bo_update(pmx_bo* h, const double* dx, double ds_max, double* p, double* s, double* rs, int32_t* status,
+                  double* q, double* bhp);
+/* y = J x for the last Jacobian (device). */
+int pmx_bo_matvec(pmx_bo* h, const double* x, double* y);
+/* Block pattern (host) and 3x3 block values (device) of the last Jacobian. */
+int pmx_bo_nnzb(pmx_bo* h, int32_t* nnzb);
+int pmx_bo_export_pattern(pmx_bo* h, int32_t* rowptr, int32_t* col);
+int pmx_bo_export_blocks(pmx_bo* h, double* vals);
+/* Kernel timing as pmx_ow_profile / pmx_ow_profile_read. */
+int pmx_bo_profile(pmx_bo* h, int32_t enable);
+int pmx_bo_profile_read(pmx_bo* h, double* out4);
+
 /* ---- incompressible TPFA pressure solve (incomp_tpfa_ad,
  *      pymrst/modules/incomp/incomp_tpfa.py:252-389) and computeTrans
  *      (pymrst/solvers/compute_trans.py:11-29) */

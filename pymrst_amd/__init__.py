@@ -19,4 +19,8 @@ def __getattr__(name):
                 "init_simple_adi_fluid", "simulate_schedule_ad", "simulateScheduleAD"):
         from . import ow
         return getattr(ow, name)
+    if name in ("ThreePhaseBlackOilModelAD", "BOState", "DeviceBOState", "DeckBlackOilFluid",
+                "init_deck_blackoil_fluid"):
+        from . import bo
+        return getattr(bo, name)
     raise AttributeError(name)

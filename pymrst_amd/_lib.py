@@ -82,6 +82,23 @@ SIGNATURES = {
                                             ctypes.POINTER(SolveInfo)]),
     "pmx_ow_prev_state_transpose": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),
     "pmx_ow_dlogk": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "pmx_bo_create": (ctypes.c_int, [_I, _P, _P, _I, _P, _I, _P, _P, _P, _I, _P, _I, _P, ctypes.POINTER(_P)]),
+    "pmx_bo_destroy": (ctypes.c_int, [_P]),
+    "pmx_bo_set_topology": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P]),
+    "pmx_bo_set_wells": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _P, _P]),
+    "pmx_bo_set_lines": (ctypes.c_int, [_P, _I, _P, _P]),
+    "pmx_bo_begin_step": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "pmx_bo_assemble": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _D]),
+    "pmx_bo_residual": (ctypes.c_int, [_P, _P]),
+    "pmx_bo_convergence": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "pmx_bo_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _D, _D, _P, ctypes.POINTER(SolveInfo)]),
+    "pmx_bo_update": (ctypes.c_int, [_P, _P, _D, _P, _P, _P, _P, _P, _P]),
+    "pmx_bo_matvec": (ctypes.c_int, [_P, _P, _P]),
+    "pmx_bo_nnzb": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "pmx_bo_export_pattern": (ctypes.c_int, [_P, _P, _P]),
+    "pmx_bo_export_blocks": (ctypes.c_int, [_P, _P]),
+    "pmx_bo_profile": (ctypes.c_int, [_P, _I]),
+    "pmx_bo_profile_read": (ctypes.c_int, [_P, _P]),
     "pmx_incomp_create": (ctypes.c_int, [_I, _P, ctypes.POINTER(_P)]),
     "pmx_incomp_destroy": (ctypes.c_int, [_P]),
     "pmx_incomp_set_grid": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P]),

@@ -7,6 +7,7 @@
 #include "../../include/pmx.h"
 #include "incomp.h"
 #include "ow.h"
+#include "bo.h"
 
 using namespace pmx;
 
@@ -18,6 +19,12 @@ struct pmx_incomp {
 struct pmx_ow {
   int device = 0;
   std::unique_ptr<OWModelDev> m;
+  pmx_solver_options last_opts{};
+};
+
+struct pmx_bo {
+  int device = 0;
+  std::unique_ptr<BOModelDev> m;
   pmx_solver_options last_opts{};
 };
 
@@ -61,6 +68,18 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
 }
 
 static void bind(pmx_ow* h) { PMX_HIP(hipSetDevice(h->device)); }
+static void bind(pmx_bo* h) { PMX_HIP(hipSetDevice(h->device)); }
+
+static void fill_info(pmx_solve_info* info, const SolveInfo& si) {
+  if (!info) return;
+  info->iterations = si.iterations;
+  info->restarts = si.restarts;
+  info->rel_residual = si.rel_residual;
+  info->setup_ms = si.setup_ms;
+  info->solve_ms = si.solve_ms;
+  info->levels = si.levels;
+  info->coarse_n = si.coarse_n;
+}
 
 extern "C" {
 
@@ -449,6 +468,183 @@ int pmx_ow_mask_residual(pmx_ow* h, const double* mask) {
     PMX_CHECK(h && mask, "null argument");
     bind(h);
     h->m->mask_residual(mask);
+  })
+}
+
+// ---------------------------------------------------------------------------
+// three-phase black-oil model
+// ---------------------------------------------------------------------------
+int pmx_bo_create(int32_t device, void* hip_stream, const double* scalars, int32_t n_pvdg, const double* pvdg,
+                  int32_t n_key, const double* keys, const int32_t* branch_ptr, const double* branch,
+                  int32_t n_swof, const double* swof, int32_t n_sgof, const double* sgof, pmx_bo** out) {
+  PMX_TRY({
+    PMX_CHECK(out && scalars && pvdg && keys && branch_ptr && branch && swof && sgof, "null argument");
+    auto h = std::make_unique<pmx_bo>();
+    h->device = device;
+    bind(h.get());
+    h->m = std::make_unique<BOModelDev>((hipStream_t)hip_stream);
+    h->m->fluid.build(scalars, n_pvdg, pvdg, n_key, keys, branch_ptr, branch, n_swof, swof, n_sgof, sgof);
+    h->m->fluid.upload(h->m->stream);
+    PMX_HIP(hipStreamSynchronize(h->m->stream));
+    pmx_solver_defaults(&h->last_opts);
+    *out = h.release();
+  })
+}
+
+int pmx_bo_destroy(pmx_bo* h) {
+  PMX_TRY({
+    if (h) {
+      bind(h);
+      delete h;
+    }
+  })
+}
+
+int pmx_bo_set_topology(pmx_bo* h, int32_t nc, int32_t nconn, const int32_t* N, const double* T, const double* gdz,
+                        const double* pv) {
+  PMX_TRY({
+    PMX_CHECK(h && N && T && gdz && pv, "null argument");
+    bind(h);
+    h->m->set_topology(nc, nconn, N, T, gdz, pv);
+  })
+}
+
+int pmx_bo_set_wells(pmx_bo* h, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells, const double* WI,
+                     const int32_t* type, const double* val, const double* compi) {
+  PMX_TRY({
+    PMX_CHECK(h && (nw == 0 || (perf_ptr && perf_cells && WI && type && val && compi)), "null argument");
+    bind(h);
+    for (int w = 0; w < nw; ++w) PMX_CHECK(type[w] == 0 || type[w] == 1, "unsupported well control");
+    h->m->set_wells(nw, perf_ptr, perf_cells, WI, type, val, compi);
+  })
+}
+
+int pmx_bo_set_lines(pmx_bo* h, int32_t nlines, const int32_t* line_ptr, const int32_t* cells) {
+  PMX_TRY({
+    PMX_CHECK(h && line_ptr && cells && nlines >= 0, "null argument");
+    cpr_set_lines(h->m->cpr, nlines, line_ptr, cells);
+  })
+}
+
+int pmx_bo_begin_step(pmx_bo* h, const double* p0, const double* s0, const double* rs0, const int32_t* status0) {
+  PMX_TRY({
+    PMX_CHECK(h && p0 && s0 && rs0 && status0, "null argument");
+    bind(h);
+    h->m->begin_step(p0, s0, rs0, status0);
+  })
+}
+
+int pmx_bo_assemble(pmx_bo* h, const double* p, const double* s, const double* rs, const int32_t* status,
+                    const double* q, const double* bhp, double dt) {
+  PMX_TRY({
+    PMX_CHECK(h && p && s && rs && status, "null argument");
+    PMX_CHECK(h->m->nw == 0 || (q && bhp), "well state required");
+    PMX_CHECK(dt > 0, "dt must be positive");
+    bind(h);
+    h->m->assemble(p, s, rs, status, q, bhp, dt);
+  })
+}
+
+int pmx_bo_residual(pmx_bo* h, double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && out, "null argument");
+    bind(h);
+    PMX_CHECK(h->m->r.n >= h->m->S.n(), "assemble first");
+    PMX_HIP(hipMemcpyAsync(out, h->m->r.p, sizeof(double) * h->m->S.n(), hipMemcpyDeviceToDevice, h->m->stream));
+  })
+}
+
+int pmx_bo_convergence(pmx_bo* h, const double* p, const double* rs, const int32_t* status, double* out12,
+                       double* wells_out) {
+  PMX_TRY({
+    PMX_CHECK(h && p && rs && status && out12, "null argument");
+    bind(h);
+    h->m->convergence(p, rs, status, out12, wells_out);
+  })
+}
+
+int pmx_bo_solve(pmx_bo* h, const pmx_solver_options* o, double pmean, double rsmean, double* dx,
+                 pmx_solve_info* info) {
+  try {
+    PMX_CHECK(h && dx, "null argument");
+    bind(h);
+    SolveInfo si;
+    LinearSolverOptions lo = to_opts(o);
+    if (o) h->last_opts = *o;
+    const int st = h->m->solve(lo, dx, si, pmean, rsmean);
+    fill_info(info, si);
+    if (st < 0) {
+      g_err = "linear solve produced a non-finite residual";
+      return -2;
+    }
+    return st;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int pmx_bo_update(pmx_bo* h, const double* dx, double ds_max, double* p, double* s, double* rs, int32_t* status,
+                  double* q, double* bhp) {
+  PMX_TRY({
+    PMX_CHECK(h && dx && p && s && rs && status, "null argument");
+    PMX_CHECK(h->m->nw == 0 || (q && bhp), "well state required");
+    bind(h);
+    h->m->update(dx, ds_max, p, s, rs, status, q, bhp);
+  })
+}
+
+int pmx_bo_matvec(pmx_bo* h, const double* x, double* y) {
+  PMX_TRY({
+    PMX_CHECK(h && x && y, "null argument");
+    bind(h);
+    DArr<double> hs;
+    hs.alloc((size_t)std::max(h->m->nw, 1) * kMaxMix);
+    system_matvec(h->m->S, x, y, hs.p, h->m->stream);
+    PMX_HIP(hipStreamSynchronize(h->m->stream));
+  })
+}
+
+int pmx_bo_nnzb(pmx_bo* h, int32_t* nnzb) {
+  PMX_TRY({
+    PMX_CHECK(h && nnzb, "null argument");
+    *nnzb = h->m->S.nnzb;
+  })
+}
+
+int pmx_bo_export_pattern(pmx_bo* h, int32_t* rowptr, int32_t* col) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    bind(h);
+    auto& S = h->m->S;
+    if (rowptr)
+      PMX_HIP(hipMemcpyAsync(rowptr, S.rowptr.p, sizeof(int) * (S.nc + 1), hipMemcpyDeviceToHost, h->m->stream));
+    if (col) PMX_HIP(hipMemcpyAsync(col, S.col.p, sizeof(int) * S.nnzb, hipMemcpyDeviceToHost, h->m->stream));
+    PMX_HIP(hipStreamSynchronize(h->m->stream));
+  })
+}
+
+int pmx_bo_export_blocks(pmx_bo* h, double* vals) {
+  PMX_TRY({
+    PMX_CHECK(h && vals, "null argument");
+    bind(h);
+    auto& S = h->m->S;
+    PMX_HIP(hipMemcpyAsync(vals, S.val.p, sizeof(double) * 9 * S.nnzb, hipMemcpyDeviceToDevice, h->m->stream));
+  })
+}
+
+int pmx_bo_profile(pmx_bo* h, int32_t enable) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    cpr_profile(h->m->cpr, enable != 0);
+  })
+}
+
+int pmx_bo_profile_read(pmx_bo* h, double* out4) {
+  PMX_TRY({
+    PMX_CHECK(h && out4, "null argument");
+    bind(h);
+    cpr_profile_read(h->m->cpr, out4);
   })
 }
 

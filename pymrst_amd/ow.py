@@ -263,6 +263,9 @@ class TwoPhaseOilWaterModelAD:
         qos = np.array([w.val * w.compi[1] if w.type == "rate" else 0.0 for w in self.wells], float)
         return OWState(np.full(self.nc, float(p0)), np.full(self.nc, float(sw0)), qws, qos, bhp)
 
+    def device_state(self, st) -> DeviceOWState:
+        return st if isinstance(st, DeviceOWState) else DeviceOWState.from_host(st, self.device)
+
     # -- device primitives -------------------------------------------------
     def assemble_device(self, st0: DeviceOWState, st: DeviceOWState, dt: float, cdp: Optional[torch.Tensor]):
         P = _lib.ptr
@@ -378,7 +381,7 @@ class TwoPhaseOilWaterModelAD:
         return st.to_host(), ok, its
 
 
-def simulate_schedule_ad(state0, model: TwoPhaseOilWaterModelAD, timesteps: Sequence[float], max_cuts: int = 6,
+def simulate_schedule_ad(state0, model, timesteps: Sequence[float], max_cuts: int = 6,
                          verbose: bool = False, observer=None, keep_device: bool = False):
     """simulate_schedule_ad (simulate_schedule_ad.py:17-80): report steps,
     halving cuts on Newton failure; returns (states, iterations per step)."""
@@ -390,7 +393,7 @@ def simulate_schedule_ad(state0, model: TwoPhaseOilWaterModelAD, timesteps: Sequ
         except Exception:
             pass
 
-    state = state0 if isinstance(state0, DeviceOWState) else DeviceOWState.from_host(state0, model.device)
+    state = model.device_state(state0)
     states: List = []
     its_per_step = np.zeros(len(timesteps), dtype=np.int64)
     for i, dt_report in enumerate(timesteps):
