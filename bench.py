@@ -104,6 +104,96 @@ def cpu_baseline(case_sample, budget_s=25.0):
                         f"same workload; {wall:.1f} s, 1 thread of {cores} available"))
 
 
+def bo_spmv_bytes(nnzb: int, nc: int) -> float:
+    """Algorithmic bytes of one 3x3-block BCSR SpMV launch: every block once
+    (72 B of FP64 values + 4 B column index), the row pointers, and the cell
+    parts of x and y once each (24 B per cell each)."""
+    return nnzb * 76.0 + 4.0 * (nc + 1) + 24.0 * nc * 2
+
+
+def bo_cpu_baseline(dims=(30, 30, 10), budget_s=12.0):
+    """The oracle black-oil Newton (forward-mode AD assembly, direct solve,
+    update and flash; single thread) on the first report step of a smaller
+    grid of the same workload family."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import scipy.sparse.linalg as spla
+    from oracle.cases import bo_model_from_case
+    from pymrst_amd.grid import DAY
+    from pymrst_amd.workloads import blackoil_field
+    c = blackoil_field(dims=dims)
+    m, s0 = bo_model_from_case(c)
+    dt = float(c.dts[0]) / 4.0
+    st, its = s0.copy(), 0
+    t0 = time.perf_counter()
+    while its < m.max_iterations and time.perf_counter() - t0 < budget_s:
+        r, J, eqs = m.residual_and_jacobian(s0, st, dt)
+        if m.converged(eqs, dt, st)[0]:
+            break
+        st = m.update_state(st, spla.spsolve(J.tocsc(), -r))
+        its += 1
+    wall = time.perf_counter() - t0
+    nc = m.nc
+    return dict(value=nc * its / wall / 1e6, unit="Mcell-Newton-its/s", cores=1, kind="port",
+                sample=(f"oracle black-oil Newton (AD assembly, scipy direct solve, flash), first {its} Newton "
+                        f"iterations of a {dt / DAY:g}-day step on {dims[0]}x{dims[1]}x{dims[2]} ({nc} cells) of "
+                        f"the same workload; {wall:.1f} s, 1 thread"))
+
+
+def blackoil_leg(steps: int = 20, cpu: bool = True):
+    """The three-phase live-oil case (100x100x20, gas cap injector, four
+    producers, `steps` steps of 15 days) with the FP64 and the FP32
+    preconditioner: whole-schedule Mcell-Newton-its/s (cuts included),
+    Newton and GMRES counts, the mixed form's pressure error against FP64,
+    and the 3x3 BCSR SpMV's achieved bandwidth."""
+    import torch
+    from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
+    from pymrst_amd.workloads import blackoil_field, build_bo_model
+    case = blackoil_field(steps=steps)
+    nc = case.num_cells
+    out, p_last = {}, {}
+    for prec in (64, 32):
+        solver = CPRSolver(rtol=1e-8, precision=prec)
+        model, s0 = build_bo_model(case, solver)
+        cuts = [0]
+
+        def progress(event, payload):
+            if event == "timestep_cut":
+                cuts[0] += 1
+            if event in ("step_accepted", "timestep_cut"):
+                print(f"[bench] black-oil FP{prec} {event} {payload}", file=sys.stderr, flush=True)
+
+        model.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        states, its = simulate_schedule_ad(s0, model, case.dts, keep_device=True, observer=progress)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        prof = model.profile_read()
+        model.profile(False)
+        newton = int(model.stats["newton"])
+        nbytes = bo_spmv_bytes(model.nnzb, nc)
+        avg_s = prof["spmv_ms"] / max(prof["spmv_count"], 1) / 1e3
+        h = states[-1].to_host()
+        p_last[prec] = h.pressure
+        out[f"fp{prec}"] = dict(value=round(nc * newton / wall / 1e6, 3), wall_s=round(wall, 3),
+                                newton_its=newton, newton_its_per_step=its.tolist(), cuts=cuts[0],
+                                gmres_its=int(sum(x[0] for x in solver.history)),
+                                spmv={"achieved_GBs": round(nbytes / avg_s / 1e9, 1) if prof["spmv_count"] else None,
+                                      "avg_launch_us": round(avg_s * 1e6, 2), "launches": prof["spmv_count"],
+                                      "bytes_per_launch": nbytes},
+                                saturated_cells_end=int((h.status == 3).sum()))
+        print(f"[bench] black-oil FP{prec}: {newton} Newton its, {wall:.2f} s", file=sys.stderr, flush=True)
+        del model
+    out["workload"] = (f"black-oil {case.dims[0]}x{case.dims[1]}x{case.dims[2]} ({nc} cells), gas-cap injector "
+                       f"1e6 Sm3/d, 4 producers at 150 bar, {steps} x 15 d, CPR-FGMRES rtol 1e-8")
+    out["max_rel_pressure_diff_mixed_vs_fp64"] = float(np.max(np.abs(p_last[32] - p_last[64]) /
+                                                              np.abs(p_last[64])))
+    out["speedup_mixed_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 3)
+    if cpu:
+        out["cpu_baseline"] = bo_cpu_baseline()
+    return out
+
+
 def adjoint_leg(fd_cells: int = 0, seed: int = 5):
     """The log-permeability adjoint on the 64x64x16 oil-water case (20 steps
     of 15 days): one forward run keeping checkpoints, one backward sweep,
@@ -271,6 +361,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-blackoil", action="store_true")
+    ap.add_argument("--blackoil-only", action="store_true")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-loop", type=int, default=16, help="members timed in the looped plain form")
@@ -292,6 +384,9 @@ def main():
         return traffic_probe(case, args.traffic_probe)
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_leg(args.ensemble_members, args.ensemble_loop)}))
+        return 0
+    if args.blackoil_only:
+        print(json.dumps({"blackoil": blackoil_leg(cpu=not args.no_cpu_baseline)}))
         return 0
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_fd)}))
@@ -344,6 +439,8 @@ def main():
                        "cells": nc, "report_steps": steps},
             "roofline": roof, "cpu_baseline": cpu}
     line.update(extra)
+    if not args.no_blackoil:
+        line["blackoil"] = blackoil_leg(cpu=not args.no_cpu_baseline)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(args.adjoint_fd)
     if not args.no_ensemble:

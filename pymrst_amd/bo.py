@@ -268,9 +268,15 @@ class ThreePhaseBlackOilModelAD:
         rs = rsat.copy() if rs0 is None else np.minimum(np.broadcast_to(np.asarray(rs0, float), (nc,)), rsat)
         status = np.where(sg > 0, 3, 1).astype(np.int64)
         s = np.column_stack([sw, 1.0 - sw - sg, sg])
-        bhp = np.array([w.val if w.type == "bhp" else float(p.mean()) for w in self.wells], float)
-        q = np.zeros(nw)
-        return BOState(p, s, rs, status, q.copy(), q.copy(), q.copy(), bhp)
+        # rate wells start at their target split by compi, with the bhp 1 bar
+        # beyond the perforated cells' pressures (every connection flows the
+        # intended way; a rate injector that starts crossflowing injects the
+        # produced oil in the first linearisation)
+        bhp = np.array([w.val if w.type == "bhp" else
+                        (p[w.cells].max() + 1e5 if w.val > 0 else p[w.cells].min() - 1e5) for w in self.wells], float)
+        q = np.array([[w.val * float(c) for c in np.asarray(w.compi, float)[:3]] if w.type == "rate" else [0.0] * 3
+                      for w in self.wells]).reshape(nw, 3)
+        return BOState(p, s, rs, status, q[:, 0].copy(), q[:, 1].copy(), q[:, 2].copy(), bhp)
 
     def device_state(self, st) -> DeviceBOState:
         return st if isinstance(st, DeviceBOState) else DeviceBOState.from_host(st, self.device)

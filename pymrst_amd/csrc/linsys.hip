@@ -1779,6 +1779,19 @@ static bool stagnated(LinearSolverOptions& lo, double now, double before, int ve
   return true;
 }
 
+// A solve whose restart cycles have not halved the best residual ten times
+// in a row will not converge within the budget: stop it (not converged), so
+// the Newton step is cut at once instead of after maxiter cycles.
+constexpr int kHopelessCycles = 10;
+static bool hopeless(double beta, double& best, int& since_best) {
+  if (beta < 0.5 * best) {
+    best = beta;
+    since_best = 0;
+    return false;
+  }
+  return ++since_best >= kHopelessCycles;
+}
+
 int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, const LinearSolverOptions& o,
               SolveInfo& info, hipStream_t s) {
   if (C->seg) return cpr_solve_segmented(C, S, b, x, o, info, s);
@@ -1812,7 +1825,8 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   }
   const double tol = o.rtol * bnorm;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hcol(m + 2), y(m);
-  double beta = bnorm, beta_cycle = bnorm;
+  double beta = bnorm, beta_cycle = bnorm, best = bnorm;
+  int since_best = 0;
   LinearSolverOptions lo = o;   // the local stage may fall back below
   // r = b
   PMX_HIP(hipMemcpyAsync(C->r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1921,6 +1935,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     ++info.restarts;
     stagnated(lo, beta, beta_cycle, o.verbose);
     beta_cycle = beta;
+    if (hopeless(beta, best, since_best)) break;
   }
   info.iterations = total;
   info.levels = (int)C->L.size();
@@ -2183,7 +2198,8 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       if (!fin[k]) return false;
     return true;
   };
-  double worst = 0.0, worst_cycle = 1.0;
+  double worst = 0.0, worst_cycle = 1.0, worst_best = 1.0;
+  int since_best = 0;
   int worst_seg = -1;
   LinearSolverOptions lo = o;   // the local stage may fall back below
   while (cycles < o.maxiter && !all_fin()) {
@@ -2327,6 +2343,7 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
     ++info.restarts;
     stagnated(lo, worst, worst_cycle, o.verbose);
     worst_cycle = worst;
+    if (hopeless(worst, worst_best, since_best)) break;
   }
   info.iterations = total;
   info.rel_residual = worst;

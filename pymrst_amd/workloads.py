@@ -142,3 +142,113 @@ def ensemble_cases(members=256, seed=3, steps=5, step_days=30.0, rate_scale=0.25
         c.p_init = c.p0 + c.gravity * rho0 * (z - 0.5 * cell_dz)
         cases.append(c)
     return cases
+
+
+@dataclass
+class BOCase:
+    dims: tuple
+    phys: tuple
+    perm: np.ndarray
+    poro: np.ndarray
+    deck: dict
+    wells: list             # (cells, type, val, compi)
+    p_init: np.ndarray
+    sw0: float
+    sg_init: np.ndarray
+    rs0: Optional[float]    # None: oil saturated at the initial pressure
+    dts: np.ndarray
+    gravity: float
+
+    @property
+    def num_cells(self):
+        return int(np.prod(self.dims))
+
+
+def synthetic_bo_deck(nrows: int = 20, p_lo=50 * BARSA, p_hi=400 * BARSA) -> dict:
+    """The synthetic monotone black-oil PVT: `nrows` saturated PVTO rows over
+    50-400 bar with Bo 1.05 -> 1.35, Rs 0 -> 150 Sm3/Sm3, mu_o 2 -> 0.8 cP,
+    each with one undersaturated row 50 bar above (Bo -1 %, mu_o +5 %); PVDG
+    with Bg from a real-gas Z factor 0.95 -> 0.85 at 373.15 K and mu_g
+    0.012 -> 0.024 cP; Corey-2 SWOF/SGOF with connate water 0.2; PVTW
+    (B 1.01 at 200 bar, c 4e-10 /Pa, 0.5 cP), ROCK c 5e-10 /Pa at 200 bar;
+    surface densities 1000 / 850 / 0.9 kg/m3."""
+    p = np.linspace(p_lo, p_hi, nrows)
+    t = (p - p_lo) / (p_hi - p_lo)
+    rs = 150.0 * t
+    bo = 1.05 + 0.30 * t
+    muo = (2.0 - 1.2 * t) * CENTI * POISE
+    pvto = [(rs[k], [[p[k], bo[k], muo[k]], [p[k] + 50 * BARSA, bo[k] * 0.99, muo[k] * 1.05]])
+            for k in range(nrows)]
+    Z = 0.95 - 0.10 * t
+    Tk, p_sc, T_sc = 373.15, 1.01325e5, 288.71
+    Bg = Z * Tk * p_sc / (p * T_sc)
+    mug = (0.012 + 0.012 * t) * CENTI * POISE
+    sw = np.linspace(0.2, 1.0, 9)
+    swof = np.column_stack([sw, ((sw - 0.2) / 0.8) ** 2, ((1.0 - sw) / 0.8) ** 2, np.zeros_like(sw)])
+    sg = np.linspace(0.0, 0.8, 9)
+    sgof = np.column_stack([sg, (sg / 0.8) ** 2, ((0.8 - sg) / 0.8) ** 2, np.zeros_like(sg)])
+    return dict(rhoS=[1000.0, 850.0, 0.9],
+                pvtw=dict(p_ref=200 * BARSA, b_ref=1.01, c=4e-10, mu_ref=0.5 * CENTI * POISE, viscosibility=0.0),
+                rock_cp=dict(p_ref=200 * BARSA, c=5e-10), pvdg=np.column_stack([p, Bg, mug]), pvto=pvto,
+                swof=swof, sgof=sgof)
+
+
+def blackoil_field(dims=(100, 100, 20), cell=(20.0, 20.0, 2.0), seed=2, steps=20, step_days=15.0,
+                   gas_rate=1.0e6 / DAY, prod_bhp=150 * BARSA, p_top=250 * BARSA, cap_layers=4, sg_cap=0.25,
+                   sigma=1.5) -> BOCase:
+    """The three-phase live-oil case: a `dims` Cartesian grid (200,000 cells
+    of 20 x 20 x 2 m by default), layered lognormal permeability as in the
+    oil-water field case (per-layer median log-uniform on 1-1000 mD,
+    in-layer sigma(ln k) = `sigma`, kz/kx = 0.1, porosity 0.05 + 0.25 Phi(z)),
+    the synthetic PVT deck and connate water.  A gas cap (gas saturation
+    `sg_cap`) fills the top `cap_layers` layers over oil saturated at its
+    pressure, in hydrostatic equilibrium from `p_top` at the top.  A centre
+    gas injector (1e6 Sm3/day) is completed in the gas cap, four corner
+    producers at 150 bar in the oil zone below the cap's first layer of
+    oil; gravity on; `steps` steps of `step_days` days.  Drawdown below the
+    bubble point liberates gas around the producers, so cells switch from
+    Rs to Sg as primary variable during the run."""
+    nx, ny, nz = dims
+    nc = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    medians_md = 10.0 ** rng.uniform(0.0, 3.0, nz)
+    z = rng.standard_normal(nc)
+    layer = np.arange(nc) // (nx * ny)
+    kx = np.exp(np.log(medians_md[layer]) + sigma * z) * MILLI * DARCY
+    perm = np.column_stack([kx, kx, 0.1 * kx])
+    poro = 0.05 + 0.25 * ndtr(z)
+    column = lambda i, j, ks: i + nx * j + nx * ny * np.asarray(ks)
+    cap = np.arange(cap_layers)
+    oil = np.arange(cap_layers + 1, nz)
+    wells = [(column(nx // 2, ny // 2, cap), "rate", gas_rate, (0.0, 0.0, 1.0))]
+    for i, j in ((0, 0), (nx - 1, 0), (0, ny - 1), (nx - 1, ny - 1)):
+        wells.append((column(i, j, oil), "bhp", prod_bhp, (0.0, 1.0, 0.0)))
+    sw0 = 0.2
+    sg_layer = np.where(np.arange(nz) < cap_layers, sg_cap, 0.0)
+    # hydrostatic column with the mixture density of each layer (oil about
+    # 750, gas about 190 kg/m3 at reservoir conditions)
+    rho = sw0 * 1000.0 + sg_layer * 190.0 + (1.0 - sw0 - sg_layer) * 750.0
+    p_layer = np.empty(nz)
+    p_layer[0] = p_top
+    for k in range(1, nz):
+        p_layer[k] = p_layer[k - 1] + NORM_GRAVITY * 0.5 * (rho[k - 1] + rho[k]) * cell[2]
+    return BOCase(dims=dims, phys=(nx * cell[0], ny * cell[1], nz * cell[2]), perm=perm, poro=poro,
+                  deck=synthetic_bo_deck(), wells=wells, p_init=p_layer[layer], sw0=sw0, sg_init=sg_layer[layer],
+                  rs0=None, dts=np.full(steps, step_days * DAY), gravity=NORM_GRAVITY)
+
+
+def build_bo_model(case: BOCase, linear_solver=None, device=None):
+    """The device black-oil model and initial state for a case."""
+    from . import grid as G
+    from .bo import ThreePhaseBlackOilModelAD, init_deck_blackoil_fluid
+    g = G.cart_grid(case.dims, case.phys)
+    rock = G.make_rock(g, case.perm, case.poro)
+    grav = np.array([0.0, 0.0, case.gravity]) if case.gravity else None
+    wells = None
+    for cells, kind, val, compi in case.wells:
+        wells = G.add_well(wells, g, rock, cells, kind, val, compi=np.array(compi),
+                           sign=1 if kind == "rate" else -1, gravity=grav)
+    model = ThreePhaseBlackOilModelAD(g, rock, init_deck_blackoil_fluid(case.deck), wells, gravity=case.gravity,
+                                      linear_solver=linear_solver, device=device)
+    st = model.initial_state(case.p_init, case.sw0, case.sg_init, rs0=case.rs0)
+    return model, st
