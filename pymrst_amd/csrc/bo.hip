@@ -655,13 +655,22 @@ __global__ __launch_bounds__(256) void k_bo_conv(int nc, const double* __restric
   }
 }
 
+// One wave per quantity: lanes stride over the block partials, then a
+// shuffle tree (fixed order, deterministic); the first 3 quantities are maxima.
 __global__ void k_bo_conv_final(int nblk, const double* __restrict__ partial, double* __restrict__ out) {
-  const int q = threadIdx.x;
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (q >= kBOConv) return;
-  double a = partial[q];
-  for (int b = 1; b < nblk; ++b)
-    a = q < 3 ? fmax(a, partial[(size_t)b * kBOConv + q]) : a + partial[(size_t)b * kBOConv + q];
-  out[q] = a;
+  const bool is_max = q < 3;
+  double a = 0.0;   // the maxima are of absolute values
+  for (int b = lane; b < nblk; b += 64) {
+    const double v = partial[(size_t)b * kBOConv + q];
+    a = is_max ? fmax(a, v) : a + v;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(a, off, 64);
+    a = is_max ? fmax(a, o) : a + o;
+  }
+  if (lane == 0) out[q] = a;
 }
 
 // Newton update with the variable switch and the flash (update_state
@@ -977,7 +986,7 @@ void BOModelDev::convergence(const double* p, const double* rs, const int* statu
                      fluid.view(), conv_partial.p);
   PMX_HIP(hipGetLastError());
   double* fin = conv_partial.p + (size_t)kBOConvBlocks * kBOConv;
-  hipLaunchKernelGGL(k_bo_conv_final, dim3(1), dim3(64), 0, stream, kBOConvBlocks, conv_partial.p, fin);
+  hipLaunchKernelGGL(k_bo_conv_final, dim3(1), dim3(64 * kBOConv), 0, stream, kBOConvBlocks, conv_partial.p, fin);
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipMemcpyAsync(out12, fin, sizeof(double) * kBOConv, hipMemcpyDeviceToHost, stream));
   if (nw > 0 && wells_out)

@@ -319,12 +319,22 @@ __global__ __launch_bounds__(256) void k_ow_conv(int nc, const double* __restric
   }
 }
 
+// One wave per quantity: lanes stride over the block partials, then a
+// shuffle tree (fixed order, deterministic); the first 2 quantities are maxima.
 __global__ void k_ow_conv_final(int nblk, const double* __restrict__ partial, double* __restrict__ out) {
-  const int q = threadIdx.x;
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (q >= 7) return;
-  double a = partial[q];
-  for (int b = 1; b < nblk; ++b) a = q < 2 ? fmax(a, partial[(size_t)b * 7 + q]) : a + partial[(size_t)b * 7 + q];
-  out[q] = a;
+  const bool is_max = q < 2;
+  double a = 0.0;   // the maxima are of absolute values
+  for (int b = lane; b < nblk; b += 64) {
+    const double v = partial[(size_t)b * 7 + q];
+    a = is_max ? fmax(a, v) : a + v;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(a, off, 64);
+    a = is_max ? fmax(a, o) : a + o;
+  }
+  if (lane == 0) out[q] = a;
 }
 
 // Newton update with the reference's limits (step :629-651): dp clipped to
@@ -465,7 +475,7 @@ void OWModelDev::convergence(const double* p, double* out7, double* wells_out) {
   hipLaunchKernelGGL(k_ow_conv, dim3(kConvBlocks), dim3(256), 0, stream, nc, r.p, p, pv.p, fluid, conv_partial.p);
   PMX_HIP(hipGetLastError());
   double* fin = conv_partial.p + (size_t)kConvBlocks * 7;
-  hipLaunchKernelGGL(k_ow_conv_final, dim3(1), dim3(64), 0, stream, kConvBlocks, conv_partial.p, fin);
+  hipLaunchKernelGGL(k_ow_conv_final, dim3(1), dim3(64 * 7), 0, stream, kConvBlocks, conv_partial.p, fin);
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipMemcpyAsync(out7, fin, sizeof(double) * 7, hipMemcpyDeviceToHost, stream));
   if (nw > 0 && wells_out)
