@@ -26,4 +26,12 @@ Pinning (what checks each part; see tests/test_oracle_*.py):
 * gravity well ``cdp`` topology walk: parity unpinned beyond the hydrostatic
   closed form (no golden vector exists; the reference path that computes it
   cannot be imported in this container).
+* black-oil (``oracle/bo.py``): MRST's per-face flux fixture
+  ``bo_phase_flux.npz`` for all three phases; closed forms of the PVT
+  'parallel' interpolation; the Jacobian against central finite differences
+  of the residual (with and without wells); conservation of water and of
+  total gas over a converged closed-box step.  The wellbore mixture, the
+  variable switch and the flash have no golden vector offline (the SPE1
+  fixtures need an absent deck): pinned only by those FD and conservation
+  checks.
 """
