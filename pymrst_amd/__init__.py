@@ -19,6 +19,9 @@ def __getattr__(name):
                 "init_simple_adi_fluid", "simulate_schedule_ad", "simulateScheduleAD"):
         from . import ow
         return getattr(ow, name)
+    if name in ("NonLinearSolver", "StepReport", "SimulationReport"):
+        from . import nonlinear
+        return getattr(nonlinear, name)
     if name in ("ThreePhaseBlackOilModelAD", "BOState", "DeviceBOState", "DeckBlackOilFluid",
                 "init_deck_blackoil_fluid"):
         from . import bo

@@ -347,6 +347,26 @@ class ThreePhaseBlackOilModelAD:
                                           P(st.status), P(st.q), P(st.bhp)), "pmx_bo_update")
         return True
 
+    # -- ministep protocol of pymrst_amd.nonlinear.NonLinearSolver --------------
+    def begin_ministep(self, state0: DeviceBOState):
+        self.begin_step(state0)
+
+    def linearize(self, state0: DeviceBOState, st: DeviceBOState, dt: float):
+        """Assemble at st and test convergence: (converged, {CNV, MB})."""
+        self.assemble_device(st, dt)
+        ok, self._ms_pmean, self._ms_rsmean = self.converged(dt, st)
+        cnv, mb = self.last_cnv_mb
+        return ok, {"CNV": float(cnv), "MB": float(mb)}
+
+    def newton_update(self, st: DeviceBOState) -> bool:
+        self.stats["newton"] += 1
+        return self.solve_and_update(st, self._ms_pmean, self._ms_rsmean)
+
+    @property
+    def last_linear_solve(self):
+        h = self.linear_solver.history[-1]
+        return int(h[0]), float(h[1])
+
     # -- Newton (step :547-571) --------------------------------------------------
     def step_device(self, state: DeviceBOState, dt: float):
         st = state.copy()

@@ -354,6 +354,26 @@ class TwoPhaseOilWaterModelAD:
         self._cdp_dev[:cdp.size].copy_(torch.as_tensor(cdp, dtype=torch.float64))
         return self._cdp_dev
 
+    # -- ministep protocol of pymrst_amd.nonlinear.NonLinearSolver --------------
+    def begin_ministep(self, state0: DeviceOWState):
+        self._ms_cdp = self.compute_cdp_per_well(state0)
+
+    def linearize(self, state0: DeviceOWState, st: DeviceOWState, dt: float):
+        """Assemble at st and test convergence: (converged, {CNV, MB})."""
+        self.assemble_device(state0, st, dt, self._ms_cdp)
+        ok, self._ms_pmean = self.converged(dt, st)
+        cnv, mb = self.last_cnv_mb
+        return ok, {"CNV": float(cnv), "MB": float(mb)}
+
+    def newton_update(self, st: DeviceOWState) -> bool:
+        self.stats["newton"] += 1
+        return self.solve_and_update(st, self._ms_pmean)
+
+    @property
+    def last_linear_solve(self):
+        h = self.linear_solver.history[-1]
+        return int(h[0]), float(h[1])
+
     # -- Newton ----------------------------------------------------------------
     def step_device(self, state: DeviceOWState, dt: float):
         """step (:605-657) on device state: returns (state, converged, its)."""

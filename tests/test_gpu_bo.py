@@ -182,3 +182,35 @@ def test_gas_injection_schedule_matches_oracle(prec):
             np.testing.assert_allclose(x, y, rtol=1e-7, atol=1e-12)
         switched |= bool((b.status == 3).any() and (b.status == 1).any())
     assert switched, "the schedule never mixes saturated and undersaturated cells"
+
+
+@pytest.mark.parametrize("maxit, days", [(4, [30.0, 90.0]), (3, [30.0, 30.0])])
+def test_nonlinear_solver_cuts_match_oracle(maxit, days):
+    """NonLinearSolver.simulate_schedule on the device model against the
+    oracle restatement of solve_timestep / _iter_timesteps: with the Newton
+    cap lowered, the first report step is taken as 2^k equal sub-steps
+    (same k, same summed Newton count, same states); with a cap too low to
+    converge at all, the run stops after the failed report step."""
+    from pymrst_amd.nonlinear import NonLinearSolver
+    from pymrst_amd.ow import CPRSolver
+    from oracle.nonlinear import simulate_schedule as oracle_schedule
+    mo, md = _pair(dims=(6, 5, 3), seed=7, wells="injector1", linear_solver=CPRSolver(rtol=1e-12))
+    mo.max_iterations = maxit
+    st0 = mo.initial_state(200 * og.BARSA, 0.25, 0.0, rs0=45.0)
+    st0.bhp[:] = [200 * og.BARSA, 150 * og.BARSA]
+    dts = np.array(days) * og.DAY
+    ref_states, ref_its, ref_cuts = oracle_schedule(mo, st0, dts)
+    events = []
+    nls = NonLinearSolver(max_iterations=maxit, observer=lambda e, p: events.append(e))
+    states, rep = nls.simulate_schedule(md, st0, dts)
+    assert [s.iterations for s in rep.steps] == ref_its.tolist()
+    assert [s.cuts for s in rep.steps] == ref_cuts.tolist()
+    assert len(states) == len(ref_states)
+    assert rep.steps[0].cuts > 0 and "timestep_cut" in events
+    for a, b, r in zip(states, ref_states, rep.steps):
+        if not r.converged:
+            assert events[-1] == "step_rejected"
+            continue
+        np.testing.assert_array_equal(a.status, b.status)
+        np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
+        np.testing.assert_allclose(a.s, b.s, rtol=0, atol=1e-8)

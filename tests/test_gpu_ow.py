@@ -218,3 +218,22 @@ def test_graph_replay_matches_eager(prec):
             torch.cuda.synchronize()
             assert torch.equal(z, eager[k]), f"replay {rep}, vector {k}: differs from the eager result"
             assert torch.equal(apply(r, False), eager[k])
+
+
+def test_nonlinear_solver_drives_the_oil_water_model():
+    """NonLinearSolver over the oil-water device model (well heads from
+    state0 per ministep) equals the model's own step when nothing is cut,
+    and the oracle's step when it is."""
+    from pymrst_amd.nonlinear import NonLinearSolver
+    from pymrst_amd.ow import CPRSolver
+    from oracle.nonlinear import simulate_schedule as oracle_schedule
+    inp = small_gravity_inputs()
+    mo, so = build_oracle(inp)
+    md, sd = build_device(inp, CPRSolver(rtol=1e-10))
+    dts = np.array([10.0, 30.0]) * og.DAY
+    ref_states, ref_its, ref_cuts = oracle_schedule(mo, so, dts)
+    states, rep = NonLinearSolver().simulate_schedule(md, sd, dts)
+    assert rep.converged and [s.iterations for s in rep.steps] == ref_its.tolist()
+    for a, b in zip(states, ref_states):
+        np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
+        np.testing.assert_allclose(a.sW, b.sW, rtol=0, atol=1e-8)
