@@ -437,6 +437,105 @@ __global__ void k_multiaxpy_scale(size_t n, int nv, const double* __restrict__ V
   w[i] = wi * scale;
 }
 
+// Two right-hand sides against the basis, for delayed reorthogonalisation:
+// partial[blk][0..nv)    = V_d . a   (a: the lagged vector in slot nv)
+// partial[blk][nv..2nv)  = V_d . w
+// partial[blk][2nv..+3)  = a . a, a . w, w . w
+// Each basis vector is read once.
+constexpr int kDot2Group = kRedGroup / 2;
+__global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, const double* __restrict__ V,
+                                                           size_t ld, const double* __restrict__ a,
+                                                           const double* __restrict__ w,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[kRedThreads / 64][kRedGroup];
+  const int nd = 2 * nv + 3;
+  const size_t base = (size_t)blockIdx.x * kRedTile + threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double av[kRedElems], wv[kRedElems];
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    const size_t i = base + (size_t)e * kRedThreads;
+    av[e] = i < n ? a[i] : 0.0;
+    wv[e] = i < n ? w[i] : 0.0;
+  }
+  for (int d0 = 0; d0 < nv; d0 += kDot2Group) {
+    double acc[kRedGroup];
+#pragma unroll
+    for (int q = 0; q < kDot2Group; ++q) {
+      const int d = d0 + q;
+      double sa = 0.0, sw = 0.0;
+      if (d < nv) {
+#pragma unroll
+        for (int e = 0; e < kRedElems; ++e) {
+          const size_t i = base + (size_t)e * kRedThreads;
+          if (i < n) {
+            const double v = V[(size_t)d * ld + i];
+            sa = fma(v, av[e], sa);
+            sw = fma(v, wv[e], sw);
+          }
+        }
+      }
+      acc[q] = wave_sum(sa, 64);
+      acc[kDot2Group + q] = wave_sum(sw, 64);
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < kRedGroup; ++q) sh[wid][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < kRedGroup) {
+      const int q = threadIdx.x % kDot2Group, side = threadIdx.x / kDot2Group;
+      if (d0 + q < nv) {
+        double t = 0.0;
+#pragma unroll
+        for (int v = 0; v < kRedThreads / 64; ++v) t += sh[v][threadIdx.x];
+        partial[(size_t)blockIdx.x * nd + side * nv + d0 + q] = t;
+      }
+    }
+    __syncthreads();
+  }
+  double aa = 0.0, aw = 0.0, ww = 0.0;
+#pragma unroll
+  for (int e = 0; e < kRedElems; ++e) {
+    aa = fma(av[e], av[e], aa);
+    aw = fma(av[e], wv[e], aw);
+    ww = fma(wv[e], wv[e], ww);
+  }
+  aa = wave_sum(aa, 64);
+  aw = wave_sum(aw, 64);
+  ww = wave_sum(ww, 64);
+  if (lane == 0) {
+    sh[wid][0] = aa;
+    sh[wid][1] = aw;
+    sh[wid][2] = ww;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < kRedThreads / 64; ++v) t += sh[v][threadIdx.x];
+    partial[(size_t)blockIdx.x * nd + 2 * nv + threadIdx.x] = t;
+  }
+}
+
+// Delayed reorthogonalisation step, one read of the basis:
+//   a <- (a - sum_d s_d V_d) * inv_nu                       (coef[0..nv))
+//   w <- (w - sum_d c_d V_d - a_coef * a_old) * scale       (coef[nv..2nv))
+__global__ void k_dcgs2_update(size_t n, int nv, const double* __restrict__ V, size_t ld,
+                               const double* __restrict__ coef, double inv_nu, double a_coef, double scale,
+                               double* __restrict__ a, double* __restrict__ w) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double av = a[i];
+  double sa = 0.0, sw = 0.0;
+  for (int d = 0; d < nv; ++d) {
+    const double v = V[(size_t)d * ld + i];
+    sa = fma(coef[d], v, sa);
+    sw = fma(coef[nv + d], v, sw);
+  }
+  a[i] = (av - sa) * inv_nu;
+  w[i] = (w[i] - sw - a_coef * av) * scale;
+}
+
 // x += sum_d coef[d] Z_d
 __global__ void k_multiaxpy_add(size_t n, int nv, const double* __restrict__ Z, size_t ld,
                                 const double* __restrict__ coef, double* __restrict__ x) {
@@ -1804,9 +1903,9 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     C->Z.alloc(ld * m);
     C->w.alloc(ld);
     C->r.alloc(ld);
-    C->partial.alloc((size_t)red_blocks(n) * (m + 2) + 2 * (m + 2));
-    C->hdev.alloc(m + 2);
-    C->ydev.alloc(m + 2);
+    C->partial.alloc((size_t)red_blocks(n) * (2 * m + 4) + 2 * (2 * m + 4));
+    C->hdev.alloc(2 * m + 4);
+    C->ydev.alloc(2 * m + 4);
     C->hmat.alloc((size_t)std::max(S.nw, 1) * kMaxMix);
     C->m_alloc = m;
     C->n_alloc = n;
@@ -1842,8 +1941,51 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     launch_check();
     std::fill(g.begin(), g.end(), 0.0);
     g[0] = beta;
-    int j = 0;
+    // Arnoldi with delayed classical Gram-Schmidt reorthogonalisation: the
+    // new vector w = J z_j is projected once against the basis; slot j+1
+    // keeps that "lagged" remainder (scaled by sig), and it is
+    // reorthogonalised one iteration later in the same reduction that
+    // projects the next w (k_multidot2 + k_dcgs2_update: two passes over
+    // the basis per iteration).  Column j of the Hessenberg matrix is
+    // completed then; a tentative residual estimate from the lagged norm
+    // decides when to complete the last column early (one extra reduction)
+    // rather than spend a preconditioner application.  With
+    // reorthogonalize = 0 this is classical Gram-Schmidt, once.
+    const bool delayed = o.reorthogonalize != 0;
+    auto rotate = [&](int col) -> double {
+      for (int i = 0; i < col; ++i) {
+        const double a = H[(size_t)i * m + col], bq = H[(size_t)(i + 1) * m + col];
+        H[(size_t)i * m + col] = cs[i] * a + sn[i] * bq;
+        H[(size_t)(i + 1) * m + col] = -sn[i] * a + cs[i] * bq;
+      }
+      const double a = H[(size_t)col * m + col], bq = H[(size_t)(col + 1) * m + col];
+      const double rr = std::hypot(a, bq);
+      cs[col] = rr > 0 ? a / rr : 1.0;
+      sn[col] = rr > 0 ? bq / rr : 0.0;
+      H[(size_t)col * m + col] = rr;
+      H[(size_t)(col + 1) * m + col] = 0.0;
+      g[col + 1] = -sn[col] * g[col];
+      g[col] = cs[col] * g[col];
+      return std::fabs(g[col + 1]);
+    };
+    auto reduce_to_host = [&](int nd, std::vector<double>& hh) {
+      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, rb, nd, C->partial.p, C->hdev.p);
+      launch_check();
+      hh.assign(nd, 0.0);
+      PMX_HIP(hipMemcpyAsync(hh.data(), C->hdev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+      PMX_HIP(hipStreamSynchronize(s));
+    };
+    // the lagged norm from Pythagoras; where it cancels, scale by ||w|| (the
+    // scale only sets the size of the stored vector)
+    auto lag_scale = [](double e2, double ww) {
+      if (e2 > 1e-12 * ww) return std::sqrt(e2);
+      return ww > 0.0 ? std::sqrt(ww) : 1.0;
+    };
+    int j = 0, k = 0;
     double resid = beta;
+    bool final = true;   // slot j holds an orthonormalised vector
+    double sig = 1.0;    // otherwise slot j = sig * (remainder of column j-1)
+    std::vector<double> hh, coef;
     for (; j < m; ++j) {
       double* vj = C->V.p + (size_t)j * ld;
       double* zj = C->Z.p + (size_t)j * ld;
@@ -1852,68 +1994,106 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       cpr_apply(C, S, vj, zj, lo, s);
       C->prec_t.stop(s);
       system_matvec(S, zj, w, hscr, s, &C->spmv_t);
-      // Classical Gram-Schmidt: h = V^T w and w.w in one pass; the norm of
-      // the projected vector follows from Pythagoras, and a second pass runs
-      // only when it shows heavy cancellation ("twice is enough").
-      const int nd = j + 2;
-      hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
-      launch_check();
-      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, rb, nd, C->partial.p, C->hdev.p);
-      launch_check();
-      std::vector<double> hh(nd, 0.0);
-      PMX_HIP(hipMemcpyAsync(hh.data(), C->hdev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
-      PMX_HIP(hipStreamSynchronize(s));
-      double hsq = 0.0;
-      for (int i = 0; i <= j; ++i) hsq += hh[i] * hh[i];
-      double wn2 = hh[j + 1] - hsq;
-      const double* coef = C->hdev.p;
-      for (int i = 0; i <= j; ++i) hcol[i] = hh[i];
-      if (o.reorthogonalize && !(wn2 > 0.5 * hh[j + 1])) {
-        hipLaunchKernelGGL(k_multiaxpy_norm, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, C->hdev.p,
-                           w, C->partial.p);
-        launch_check();
+      ++total;
+      double e2, ww, est;
+      if (final) {
         hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
         launch_check();
-        hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, rb, nd, C->partial.p, C->ydev.p);
-        launch_check();
-        std::vector<double> h2(nd, 0.0);
-        PMX_HIP(hipMemcpyAsync(h2.data(), C->ydev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
-        PMX_HIP(hipStreamSynchronize(s));
-        double h2sq = 0.0;
+        reduce_to_host(j + 2, hh);
+        double hsq = 0.0;
         for (int i = 0; i <= j; ++i) {
-          hcol[i] += h2[i];
-          h2sq += h2[i] * h2[i];
+          H[(size_t)i * m + j] = hh[i];
+          hsq += hh[i] * hh[i];
         }
-        wn2 = h2[j + 1] - h2sq;
-        coef = C->ydev.p;
+        ww = hh[j + 1];
+        e2 = ww - hsq;
+        if (!std::isfinite(ww)) { status = -1; break; }
+        est = lag_scale(e2, ww);
+        hipLaunchKernelGGL(k_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, j + 1, C->V.p, ld, C->hdev.p,
+                           1.0 / est, w);
+        launch_check();
+      } else {
+        const int nd = 2 * j + 3;
+        hipLaunchKernelGGL(k_multidot2, dim3(rb), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
+                           C->partial.p);
+        launch_check();
+        reduce_to_host(nd, hh);
+        const double* sv = hh.data();
+        const double* tv = hh.data() + j;
+        const double alpha = hh[2 * j], bp = hh[2 * j + 1];
+        ww = hh[2 * j + 2];
+        if (!std::isfinite(ww) || !std::isfinite(alpha)) { status = -1; break; }
+        double ss = 0.0, st = 0.0, tt = 0.0;
+        for (int i = 0; i < j; ++i) {
+          ss += sv[i] * sv[i];
+          st += sv[i] * tv[i];
+          tt += tv[i] * tv[i];
+        }
+        const double nu = std::sqrt(std::max(alpha - ss, 0.0));
+        // complete column j-1: its remainder is slot j / sig = (V s + nu v_j) / sig
+        for (int i = 0; i < j; ++i) H[(size_t)i * m + j - 1] += sv[i] / sig;
+        H[(size_t)j * m + j - 1] = nu / sig;
         ++info.reorthogonalizations;
+        resid = rotate(j - 1);
+        if (resid <= tol || !(nu > 0.0)) { k = j; break; }   // this w is not needed
+        const double hjj = (bp - st) / nu;
+        for (int i = 0; i < j; ++i) H[(size_t)i * m + j] = tv[i];
+        H[(size_t)j * m + j] = hjj;
+        e2 = ww - tt - hjj * hjj;
+        est = lag_scale(e2, ww);
+        coef.resize(2 * (size_t)j);
+        for (int i = 0; i < j; ++i) {
+          coef[i] = sv[i];
+          coef[j + i] = tv[i] - sv[i] * hjj / nu;
+        }
+        PMX_HIP(hipMemcpyAsync(C->ydev.p, coef.data(), sizeof(double) * 2 * j, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_dcgs2_update, dim3(nb), dim3(256), 0, s, n, j, C->V.p, ld, C->ydev.p, 1.0 / nu,
+                           hjj / nu, 1.0 / est, vj, w);
+        launch_check();
       }
-      hcol[j + 1] = std::sqrt(std::max(wn2, 0.0));
-      ++total;
-      if (!std::isfinite(hcol[j + 1])) { status = -1; break; }
-      // apply previous rotations
+      H[(size_t)(j + 1) * m + j] = est;
+      if (!delayed) {
+        // classical Gram-Schmidt once: est is taken as the remainder's norm
+        resid = rotate(j);
+        if (o.verbose > 1) fprintf(stderr, "[pmx] fgmres it %d  rel %.3e\n", total, resid / bnorm);
+        k = j + 1;
+        if (resid <= tol || !(e2 > 0.0)) break;
+        continue;
+      }
+      sig = 1.0 / est;
+      final = false;
+      // tentative residual with the lagged norm
+      for (int i = 0; i <= j + 1; ++i) hcol[i] = H[(size_t)i * m + j];
       for (int i = 0; i < j; ++i) {
-        const double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
-        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
-        hcol[i] = t;
+        const double a = hcol[i], bq = hcol[i + 1];
+        hcol[i] = cs[i] * a + sn[i] * bq;
+        hcol[i + 1] = -sn[i] * a + cs[i] * bq;
       }
-      const double a = hcol[j], bb = hcol[j + 1];
-      const double rr = std::hypot(a, bb);
-      cs[j] = rr > 0 ? a / rr : 1.0;
-      sn[j] = rr > 0 ? bb / rr : 0.0;
-      hcol[j] = rr;
-      hcol[j + 1] = 0.0;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = hcol[i];
-      resid = std::fabs(g[j + 1]);
-      const double hnext = std::sqrt(std::max(wn2, 0.0));
-      if (o.verbose > 1) fprintf(stderr, "[pmx] fgmres it %d  rel %.3e\n", total, resid / bnorm);
-      if (resid <= tol || !(hnext > 0.0)) { ++j; break; }
-      // V_{j+1} = (w - V h) / ||w - V h||
-      hipLaunchKernelGGL(k_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, j + 1, C->V.p, ld, coef, 1.0 / hnext, w);
-      launch_check();
+      const double rr = std::hypot(hcol[j], hcol[j + 1]);
+      const double tent = rr > 0 ? std::fabs(hcol[j + 1] / rr * g[j]) : 0.0;
+      if (o.verbose > 1) fprintf(stderr, "[pmx] fgmres it %d  rel %.3e (lagged)\n", total, tent / bnorm);
+      if (tent <= tol || j == m - 1) {
+        // complete column j now: reorthogonalise slot j+1 by itself
+        hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
+        launch_check();
+        reduce_to_host(j + 2, hh);
+        double ss = 0.0;
+        for (int i = 0; i <= j; ++i) ss += hh[i] * hh[i];
+        const double nu = std::sqrt(std::max(hh[j + 1] - ss, 0.0));
+        for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] += hh[i] / sig;
+        H[(size_t)(j + 1) * m + j] = nu / sig;
+        ++info.reorthogonalizations;
+        resid = rotate(j);
+        k = j + 1;
+        if (resid <= tol || j == m - 1 || !(nu > 0.0)) break;
+        hipLaunchKernelGGL(k_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, j + 1, C->V.p, ld, C->hdev.p,
+                           1.0 / nu, w);
+        launch_check();
+        final = true;
+        sig = 1.0;
+      }
     }
+    j = k;
     if (status == -1) break;
     // back-substitution y = R^{-1} g
     for (int i = j - 1; i >= 0; --i) {
