@@ -243,7 +243,7 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     looped member by member over the first `loop_sample` members (a time
     box; the per-member rate is what the metric compares)."""
     import torch
-    from pymrst_amd.ensemble import build_ensemble
+    from pymrst_amd.ensemble import build_ensemble, simulate_ensemble
     from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
     from pymrst_amd.workloads import build_model, ensemble_cases
     cases = ensemble_cases(members)
@@ -259,27 +259,28 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
           f"{build_s:.1f} s", file=sys.stderr, flush=True)
 
     def progress(event, payload):
-        if event in ("step_accepted", "step_rejected", "timestep_cut"):
+        if event in ("step_accepted", "step_rejected"):
             print(f"[bench] ensemble {event} {payload}", file=sys.stderr, flush=True)
 
+    # per-member time stepping: every member cuts and regrows on its own
     if warmup:
-        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True, observer=progress)
+        simulate_ensemble(model, s0, dts[:warmup], keep_device=True, observer=progress)
     model.reset()
     solver.history.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    states, its = simulate_schedule_ad(s0, model, dts, keep_device=True, observer=progress)
+    states, its, member_its = simulate_ensemble(model, s0, dts, keep_device=True, observer=progress)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ncells = model.nc
     member_cells = np.array([b - a for a, b in ranges])
-    useful = int((member_cells * model.member_its).sum())
+    per_member = member_its.sum(axis=0)
+    useful = int((member_cells * per_member).sum())
     batched = {"value": round(useful / wall / 1e6, 3), "wall_s": round(wall, 3),
-               "value_all_cells_every_iteration": round(ncells * int(its.sum()) / wall / 1e6, 3),
                "newton_its_union": its.tolist(),
-               "newton_its_per_member_mean": round(float(model.member_its.mean()), 2),
+               "newton_its_per_member_mean": round(float(per_member.mean()), 2),
                "gmres_its": int(sum(h[0] for h in solver.history)), "union_build_s": round(build_s, 1)}
-    model_its_sample = model.member_its[:loop_sample].copy()
+    model_its_sample = per_member[:loop_sample].copy()
     del model, states
     torch.cuda.empty_cache()
     tot_cells_its, tot_wall, ncl, loop_its = 0, 0.0, 0, []
@@ -309,8 +310,8 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
           file=sys.stderr, flush=True)
     return {"workload": f"{members} oil-water realisations, nx,ny~U{{20..64}}, nz~U{{5..16}}, "
                         f"sigma(ln k)~U(0.5,2), 5 x 30 d",
-            "metric_note": "cell-Newton-iterations of each member while it is unconverged (converged members "
-                           "are frozen) per wall second",
+            "metric_note": "cell-Newton-iterations of each member (each with its own time steps; members "
+                           "that have converged or reached the report time are frozen) per wall second",
             "members": members, "cells": ncells,
             "batched_fp32_precond": batched, "looped_plain_fp64_sample": looped,
             "speedup_batched_vs_looped_fp64": round(batched["value"] / looped["value"], 2),

@@ -145,6 +145,11 @@ int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const 
  * zero rows of members that have converged, so their Newton update is zero
  * while the others iterate.  No reference counterpart. */
 int pmx_ow_mask_residual(pmx_ow* h, const double* mask);
+/* Per-cell step lengths for the accumulation terms of the following
+ * pmx_ow_assemble calls (device array of nc values, caller-owned, read at
+ * every assembly), so that ensemble members advance with their own time
+ * steps; NULL returns to the scalar dt.  No reference counterpart. */
+int pmx_ow_set_cell_dt(pmx_ow* h, const double* dt_cell);
 
 /* ---- discrete adjoint (pymrst/modules/optimization/adjoint.py:73-225) */
 /* Per connection f (host, 2*nconn): T_f/T_h at N0 and at N1, the derivative

@@ -463,6 +463,13 @@ int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const 
   })
 }
 
+int pmx_ow_set_cell_dt(pmx_ow* h, const double* dt_cell) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    h->m->dt_cell = dt_cell;
+  })
+}
+
 int pmx_ow_mask_residual(pmx_ow* h, const double* mask) {
   PMX_TRY({
     PMX_CHECK(h && mask, "null argument");

@@ -443,20 +443,28 @@ __global__ void k_multiaxpy_scale(size_t n, int nv, const double* __restrict__ V
 // partial[blk][2nv..+3)  = a . a, a . w, w . w
 // Each basis vector is read once.
 constexpr int kDot2Group = kRedGroup / 2;
+// Tiles are uniform (kRedTile, the last one short) when tstart is null, else
+// tile b covers [tstart[b], tstart[b] + tlen[b]) (segments: no tile straddles
+// two members).
 __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, const double* __restrict__ V,
                                                            size_t ld, const double* __restrict__ a,
                                                            const double* __restrict__ w,
-                                                           double* __restrict__ partial) {
+                                                           double* __restrict__ partial,
+                                                           const int* __restrict__ tstart,
+                                                           const int* __restrict__ tlen) {
   __shared__ double sh[kRedThreads / 64][kRedGroup];
   const int nd = 2 * nv + 3;
-  const size_t base = (size_t)blockIdx.x * kRedTile + threadIdx.x;
+  const size_t t0 = tstart ? (size_t)tstart[blockIdx.x] : (size_t)blockIdx.x * kRedTile;
+  const size_t len = tstart ? (size_t)tlen[blockIdx.x] : (n - t0 < (size_t)kRedTile ? n - t0 : (size_t)kRedTile);
+  const size_t base = t0 + threadIdx.x;
+  const size_t end = t0 + len;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double av[kRedElems], wv[kRedElems];
 #pragma unroll
   for (int e = 0; e < kRedElems; ++e) {
     const size_t i = base + (size_t)e * kRedThreads;
-    av[e] = i < n ? a[i] : 0.0;
-    wv[e] = i < n ? w[i] : 0.0;
+    av[e] = i < end ? a[i] : 0.0;
+    wv[e] = i < end ? w[i] : 0.0;
   }
   for (int d0 = 0; d0 < nv; d0 += kDot2Group) {
     double acc[kRedGroup];
@@ -468,7 +476,7 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, con
 #pragma unroll
         for (int e = 0; e < kRedElems; ++e) {
           const size_t i = base + (size_t)e * kRedThreads;
-          if (i < n) {
+          if (i < end) {
             const double v = V[(size_t)d * ld + i];
             sa = fma(v, av[e], sa);
             sw = fma(v, wv[e], sw);
@@ -2015,7 +2023,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       } else {
         const int nd = 2 * j + 3;
         hipLaunchKernelGGL(k_multidot2, dim3(rb), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
-                           C->partial.p);
+                           C->partial.p, nullptr, nullptr);
         launch_check();
         reduce_to_host(nd, hh);
         const double* sv = hh.data();
@@ -2265,6 +2273,27 @@ __global__ void k_seg_multiaxpy_add(size_t n, const int* __restrict__ seg_of, in
   x[i] = xi;
 }
 
+// Delayed reorthogonalisation step per segment (k_dcgs2_update with the
+// coefficients of the element's segment: s at coef[seg*ldc], c at
+// coef[seg*ldc + nv], and scalars sc[seg] = (1/nu, h_jj/nu, 1/est)).
+__global__ void k_seg_dcgs2_update(size_t n, const int* __restrict__ seg_of, int nv, const double* __restrict__ V,
+                                   size_t ld, const double* __restrict__ coef, int ldc,
+                                   const double* __restrict__ sc, double* __restrict__ a, double* __restrict__ w) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sg = seg_of[i];
+  const double* cf = coef + (size_t)sg * ldc;
+  const double av = a[i];
+  double sa = 0.0, sw = 0.0;
+  for (int d = 0; d < nv; ++d) {
+    const double v = V[(size_t)d * ld + i];
+    sa = fma(cf[d], v, sa);
+    sw = fma(cf[nv + d], v, sw);
+  }
+  a[i] = (av - sa) * sc[3 * sg];
+  w[i] = (w[i] - sw - sc[3 * sg + 1] * av) * sc[3 * sg + 2];
+}
+
 __global__ void k_seg_scale_copy(size_t n, const int* __restrict__ seg_of, const double* __restrict__ scale,
                                  const double* __restrict__ x, double* __restrict__ y) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2273,7 +2302,7 @@ __global__ void k_seg_scale_copy(size_t n, const int* __restrict__ seg_of, const
 
 struct SegFGMRES {
   SegmentPlan plan;
-  DArr<double> partial, dots, coef, scale;
+  DArr<double> partial, dots, coef, scale, scal3;
 };
 
 void cpr_set_segments(CprSolver* C, int nseg, const int* cell_ptr, const int* well_ptr, int B, int M, int nc,
@@ -2335,10 +2364,12 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
     C->n_alloc = n;
     C->ld = ld;
   }
-  F.partial.alloc((size_t)P.ntile * (m + 2));
-  F.dots.alloc((size_t)ns * (m + 2));
-  F.coef.alloc((size_t)ns * (m + 1));
+  const int ldc = 2 * m + 2;   // per-segment coefficient stride
+  F.partial.alloc((size_t)P.ntile * (2 * m + 4));
+  F.dots.alloc((size_t)ns * (2 * m + 4));
+  F.coef.alloc((size_t)ns * ldc);
   F.scale.alloc(ns);
+  F.scal3.alloc((size_t)ns * 3);
   double* hscr = C->hmat.p;
   const int nb = grid_for(n);
   auto seg_dots = [&](int nv, const double* w, std::vector<double>& out) {
@@ -2370,7 +2401,7 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
   }
   PMX_HIP(hipMemcpyAsync(C->r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   std::vector<double> H((size_t)ns * (m + 1) * m), cs((size_t)ns * m), sn((size_t)ns * m), g((size_t)ns * (m + 1)),
-      y((size_t)ns * m), hscale(ns), hcoef((size_t)ns * (m + 1));
+      y((size_t)ns * m), hscale(ns), hcoef((size_t)ns * ldc);
   std::vector<int> kdone(ns);
   int total = 0, status = 1, cycles = 0;
   auto all_fin = [&]() {
@@ -2395,6 +2426,40 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       g[(size_t)k * (m + 1)] = beta[k];
       kdone[k] = 0;
     }
+    // Delayed classical Gram-Schmidt per segment (as cpr_solve): slot j+1
+    // keeps the lagged remainder of column j, reorthogonalised in the next
+    // iteration's reduction, where column j is completed and each segment's
+    // convergence is tested.  A segment that converges stops; its slots are
+    // zeroed.  The last column of a cycle is completed by one extra
+    // reduction.
+    const bool delayed = o.reorthogonalize != 0;
+    auto rotate = [&](int k, int col) -> double {
+      double* Hk = &H[(size_t)k * (m + 1) * m];
+      double* csk = &cs[(size_t)k * m];
+      double* snk = &sn[(size_t)k * m];
+      double* gk = &g[(size_t)k * (m + 1)];
+      for (int i = 0; i < col; ++i) {
+        const double a = Hk[(size_t)i * m + col], bq = Hk[(size_t)(i + 1) * m + col];
+        Hk[(size_t)i * m + col] = csk[i] * a + snk[i] * bq;
+        Hk[(size_t)(i + 1) * m + col] = -snk[i] * a + csk[i] * bq;
+      }
+      const double a = Hk[(size_t)col * m + col], bq = Hk[(size_t)(col + 1) * m + col];
+      const double rr = std::hypot(a, bq);
+      csk[col] = rr > 0 ? a / rr : 1.0;
+      snk[col] = rr > 0 ? bq / rr : 0.0;
+      Hk[(size_t)col * m + col] = rr;
+      Hk[(size_t)(col + 1) * m + col] = 0.0;
+      gk[col + 1] = -snk[col] * gk[col];
+      gk[col] = csk[col] * gk[col];
+      return std::fabs(gk[col + 1]);
+    };
+    auto lag_scale = [](double e2, double ww) {
+      if (e2 > 1e-12 * ww) return std::sqrt(e2);
+      return ww > 0.0 ? std::sqrt(ww) : 1.0;
+    };
+    auto Hs = [&](int k, int i, int c) -> double& { return H[(size_t)k * (m + 1) * m + (size_t)i * m + c]; };
+    std::vector<double> sig(ns, 1.0), sc3((size_t)ns * 3);
+    bool final = true;
     int j = 0;
     for (; j < m; ++j) {
       double* vj = C->V.p + (size_t)j * ld;
@@ -2404,87 +2469,128 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       cpr_apply(C, S, vj, zj, lo, s);
       C->prec_t.stop(s);
       system_matvec(S, zj, w, hscr, s, &C->spmv_t);
-      const int nd = j + 2;
-      std::vector<double> hh;
-      seg_dots(j + 1, w, hh);
-      std::vector<double> hcol((size_t)ns * (j + 2)), wn2(ns);
-      bool reorth = false;
-      for (int k = 0; k < ns; ++k) {
-        double hsq = 0.0;
-        for (int i = 0; i <= j; ++i) {
-          const double h = active[k] ? hh[(size_t)k * nd + i] : 0.0;
-          hcol[(size_t)k * (j + 2) + i] = h;
-          hsq += h * h;
-        }
-        wn2[k] = hh[(size_t)k * nd + j + 1] - hsq;
-        if (active[k] && o.reorthogonalize && !(wn2[k] > 0.5 * hh[(size_t)k * nd + j + 1])) reorth = true;
-      }
-      // coefficients of the first projection (zero for inactive segments)
-      for (int k = 0; k < ns; ++k)
-        for (int i = 0; i <= j; ++i) hcoef[(size_t)k * (m + 1) + i] = hcol[(size_t)k * (j + 2) + i];
-      PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * (m + 1), hipMemcpyHostToDevice, s));
-      if (reorth) {
-        hipLaunchKernelGGL(k_seg_multiaxpy_norm, dim3(P.ntile), dim3(kRedThreads), 0, s, P.ntile, P.tile_start.p,
-                           P.tile_len.p, P.seg_of.p, j + 1, C->V.p, ld, F.coef.p, m + 1, w, F.partial.p);
-        launch_check();
-        std::vector<double> h2;
-        seg_dots(j + 1, w, h2);
-        for (int k = 0; k < ns; ++k) {
-          double h2sq = 0.0;
-          for (int i = 0; i <= j; ++i) {
-            const double h = active[k] ? h2[(size_t)k * nd + i] : 0.0;
-            hcol[(size_t)k * (j + 2) + i] += h;
-            hcoef[(size_t)k * (m + 1) + i] = h;
-            h2sq += h * h;
-          }
-          wn2[k] = h2[(size_t)k * nd + j + 1] - h2sq;
-        }
-        PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * (m + 1), hipMemcpyHostToDevice, s));
-        ++info.reorthogonalizations;
-      }
       ++total;
       bool any_active = false;
-      for (int k = 0; k < ns; ++k) {
-        hscale[k] = 0.0;
-        if (!active[k]) continue;
-        double* hc = &hcol[(size_t)k * (j + 2)];
-        const double hnext = std::sqrt(std::max(wn2[k], 0.0));
-        hc[j + 1] = hnext;
-        if (!std::isfinite(hnext)) return -1;
-        double* csk = &cs[(size_t)k * m];
-        double* snk = &sn[(size_t)k * m];
-        double* gk = &g[(size_t)k * (m + 1)];
-        for (int i = 0; i < j; ++i) {
-          const double t = csk[i] * hc[i] + snk[i] * hc[i + 1];
-          hc[i + 1] = -snk[i] * hc[i] + csk[i] * hc[i + 1];
-          hc[i] = t;
-        }
-        const double a = hc[j], bb = hc[j + 1];
-        const double rr = std::hypot(a, bb);
-        csk[j] = rr > 0 ? a / rr : 1.0;
-        snk[j] = rr > 0 ? bb / rr : 0.0;
-        hc[j] = rr;
-        gk[j + 1] = -snk[j] * gk[j];
-        gk[j] = csk[j] * gk[j];
-        for (int i = 0; i <= j; ++i) H[(size_t)k * (m + 1) * m + (size_t)i * m + j] = hc[i];
-        kdone[k] = j + 1;
-        if (std::fabs(gk[j + 1]) <= tol[k] || !(hnext > 0.0)) {
-          active[k] = 0;   // this segment's cycle ends here
-        } else {
-          hscale[k] = 1.0 / hnext;
+      std::fill(hcoef.begin(), hcoef.end(), 0.0);
+      if (final || !delayed) {
+        const int nd = j + 2;
+        std::vector<double> hh;
+        seg_dots(j + 1, w, hh);
+        for (int k = 0; k < ns; ++k) {
+          hscale[k] = 0.0;
+          if (!active[k]) continue;
+          double hsq = 0.0;
+          for (int i = 0; i <= j; ++i) {
+            const double h = hh[(size_t)k * nd + i];
+            Hs(k, i, j) = h;
+            hcoef[(size_t)k * ldc + i] = h;
+            hsq += h * h;
+          }
+          const double ww = hh[(size_t)k * nd + j + 1];
+          if (!std::isfinite(ww)) return -1;
+          const double e2 = ww - hsq, est = lag_scale(e2, ww);
+          Hs(k, j + 1, j) = est;
+          hscale[k] = 1.0 / est;
+          if (!delayed) {
+            // classical Gram-Schmidt once: est is the remainder's norm
+            kdone[k] = j + 1;
+            if (rotate(k, j) <= tol[k] || !(e2 > 0.0)) {
+              active[k] = 0;
+              hscale[k] = 0.0;
+              continue;
+            }
+          } else {
+            sig[k] = 1.0 / est;
+          }
           any_active = true;
         }
+        if (!any_active) {
+          ++j;
+          break;
+        }
+        PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * ldc, hipMemcpyHostToDevice, s));
+        PMX_HIP(hipMemcpyAsync(F.scale.p, hscale.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_seg_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, j + 1, C->V.p, ld,
+                           F.coef.p, ldc, F.scale.p, w);
+        launch_check();
+        if (delayed) final = false;
+      } else {
+        const int nd = 2 * j + 3;
+        hipLaunchKernelGGL(k_multidot2, dim3(P.ntile), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
+                           F.partial.p, P.tile_start.p, P.tile_len.p);
+        launch_check();
+        hipLaunchKernelGGL(k_seg_reduce, dim3(ns), dim3(64), 0, s, ns, nd, P.tptr.p, P.tidx.p, F.partial.p,
+                           F.dots.p);
+        launch_check();
+        std::vector<double> hh((size_t)ns * nd);
+        PMX_HIP(hipMemcpyAsync(hh.data(), F.dots.p, sizeof(double) * ns * nd, hipMemcpyDeviceToHost, s));
+        PMX_HIP(hipStreamSynchronize(s));
+        ++info.reorthogonalizations;
+        for (int k = 0; k < ns; ++k) {
+          double* s3 = &sc3[(size_t)k * 3];
+          s3[0] = s3[1] = s3[2] = 0.0;
+          if (!active[k]) continue;
+          const double* sv = &hh[(size_t)k * nd];
+          const double* tv = sv + j;
+          const double alpha = sv[2 * j], bp = sv[2 * j + 1], ww = sv[2 * j + 2];
+          if (!std::isfinite(ww) || !std::isfinite(alpha)) return -1;
+          double ss = 0.0, st = 0.0, tt = 0.0;
+          for (int i = 0; i < j; ++i) {
+            ss += sv[i] * sv[i];
+            st += sv[i] * tv[i];
+            tt += tv[i] * tv[i];
+          }
+          const double nu = std::sqrt(std::max(alpha - ss, 0.0));
+          for (int i = 0; i < j; ++i) Hs(k, i, j - 1) += sv[i] / sig[k];
+          Hs(k, j, j - 1) = nu / sig[k];
+          kdone[k] = j;
+          if (rotate(k, j - 1) <= tol[k] || !(nu > 0.0)) {
+            active[k] = 0;   // converged on column j-1; its slots are zeroed below
+            continue;
+          }
+          const double hjj = (bp - st) / nu;
+          for (int i = 0; i < j; ++i) Hs(k, i, j) = tv[i];
+          Hs(k, j, j) = hjj;
+          const double est = lag_scale(ww - tt - hjj * hjj, ww);
+          Hs(k, j + 1, j) = est;
+          sig[k] = 1.0 / est;
+          for (int i = 0; i < j; ++i) {
+            hcoef[(size_t)k * ldc + i] = sv[i];
+            hcoef[(size_t)k * ldc + j + i] = tv[i] - sv[i] * hjj / nu;
+          }
+          s3[0] = 1.0 / nu;
+          s3[1] = hjj / nu;
+          s3[2] = 1.0 / est;
+          any_active = true;
+        }
+        if (!any_active) break;
+        PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * ldc, hipMemcpyHostToDevice, s));
+        PMX_HIP(hipMemcpyAsync(F.scal3.p, sc3.data(), sizeof(double) * ns * 3, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_seg_dcgs2_update, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, j, C->V.p, ld, F.coef.p,
+                           ldc, F.scal3.p, vj, w);
+        launch_check();
       }
-      if (!any_active) {
+      if (delayed && j == m - 1) {
+        // complete the cycle's last column for the segments still active
+        std::vector<double> hh;
+        seg_dots(j + 1, w, hh);
+        const int nd = j + 2;
+        for (int k = 0; k < ns; ++k) {
+          if (!active[k]) continue;
+          double ss = 0.0;
+          for (int i = 0; i <= j; ++i) ss += hh[(size_t)k * nd + i] * hh[(size_t)k * nd + i];
+          const double nu = std::sqrt(std::max(hh[(size_t)k * nd + j + 1] - ss, 0.0));
+          for (int i = 0; i <= j; ++i) Hs(k, i, j) += hh[(size_t)k * nd + i] / sig[k];
+          Hs(k, j + 1, j) = nu / sig[k];
+          rotate(k, j);
+          kdone[k] = j + 1;
+        }
+        ++info.reorthogonalizations;
         ++j;
         break;
       }
-      // V_{j+1} = (w - V h) / ||w - V h|| per segment (zero for the finished)
-      PMX_HIP(hipMemcpyAsync(F.scale.p, hscale.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_seg_multiaxpy_scale, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, j + 1, C->V.p, ld,
-                         F.coef.p, m + 1, F.scale.p, w);
-      launch_check();
     }
+    for (int k = 0; k < ns; ++k) j = std::max(j, kdone[k]);
     // y_s = R_s^{-1} g_s over each segment's own iterations
     std::fill(hcoef.begin(), hcoef.end(), 0.0);
     for (int k = 0; k < ns; ++k) {
@@ -2492,13 +2598,13 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
       const double* Hk = &H[(size_t)k * (m + 1) * m];
       for (int i = jk - 1; i >= 0; --i) {
         double t = g[(size_t)k * (m + 1) + i];
-        for (int q = i + 1; q < jk; ++q) t -= Hk[(size_t)i * m + q] * hcoef[(size_t)k * (m + 1) + q];
-        hcoef[(size_t)k * (m + 1) + i] = t / Hk[(size_t)i * m + i];
+        for (int q = i + 1; q < jk; ++q) t -= Hk[(size_t)i * m + q] * hcoef[(size_t)k * ldc + q];
+        hcoef[(size_t)k * ldc + i] = t / Hk[(size_t)i * m + i];
       }
     }
-    PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * (m + 1), hipMemcpyHostToDevice, s));
+    PMX_HIP(hipMemcpyAsync(F.coef.p, hcoef.data(), sizeof(double) * ns * ldc, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_seg_multiaxpy_add, dim3(nb), dim3(256), 0, s, n, P.seg_of.p, j, C->Z.p, ld, F.coef.p,
-                       m + 1, x);
+                       ldc, x);
     launch_check();
     // true residuals
     system_matvec(S, x, C->r.p, hscr, s);

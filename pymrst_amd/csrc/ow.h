@@ -26,6 +26,7 @@ class OWModelDev {
   void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
               double* qos, double* bhp);
   void mask_residual(const double* mask);   // r *= mask (device, per unknown)
+  const double* dt_cell = nullptr;          // per-cell step length (device, caller-owned), or the scalar dt
 
   // adjoint (adjoint.hip): transposed solve of the last assembled Jacobian,
   // the previous-state coupling and the log-permeability derivative

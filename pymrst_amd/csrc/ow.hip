@@ -30,8 +30,8 @@ __global__ __launch_bounds__(256) void k_ow_cells(int nc, const int* __restrict_
                                                   const double* __restrict__ bG, const double* __restrict__ p,
                                                   const double* __restrict__ s, const double* __restrict__ p0,
                                                   const double* __restrict__ s0, const double* __restrict__ pv,
-                                                  OWFluid f, double inv_dt, double* __restrict__ r,
-                                                  double* __restrict__ J) {
+                                                  OWFluid f, double inv_dt, const double* __restrict__ dt_cell,
+                                                  double* __restrict__ r, double* __restrict__ J) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = t / LPR, lane = t % LPR;
   const bool active = row < nc;
@@ -88,6 +88,7 @@ __global__ __launch_bounds__(256) void k_ow_cells(int nc, const int* __restrict_
     }
     const double pvm = v * m, pvm0 = v * m0;
     const double so = 1.0 - sr;
+    if (dt_cell) inv_dt = 1.0 / dt_cell[row];   // per-cell step (ensemble members)
     const double aw = (pvm * P.bw * sr - pvm0 * bw0 * ss0) * inv_dt;
     const double ao = (pvm * P.bo * so - pvm0 * bo0 * (1.0 - ss0)) * inv_dt;
     r[2 * (size_t)row] = aw + fw;
@@ -450,7 +451,7 @@ void OWModelDev::assemble(const double* p, const double* s, const double* qws, c
   r.alloc(S.n());
   constexpr int LPR = 8;
   hipLaunchKernelGGL((k_ow_cells<LPR>), dim3(grid_for((size_t)nc * LPR)), dim3(256), 0, stream, nc, S.rowptr.p,
-                     S.col.p, S.diag.p, bconn.p, bT.p, bG.p, p, s, p0, s0, pv.p, fluid, 1.0 / dt, r.p, S.val.p);
+                     S.col.p, S.diag.p, bconn.p, bT.p, bG.p, p, s, p0, s0, pv.p, fluid, 1.0 / dt, dt_cell, r.p, S.val.p);
   PMX_HIP(hipGetLastError());
   if (nw > 0) {
     const double* c = cdp_dev ? cdp_dev : cdp.p;

@@ -10,8 +10,15 @@ polynomial has to damp every member's outlying eigenvalues.  Newton runs on the 
 iteration is taken while any member is unconverged, and convergence is
 judged member by member with the reference's CNV / MB / well tests
 (two_phase_oil_water_model.py _converged :549-567), so a member is never
-declared converged on the strength of the others.  A time-step cut halves
-the step of the whole ensemble.
+declared converged on the strength of the others.
+
+simulate_ensemble gives every member its own time stepping: each member
+has its own ministep length (per-cell step lengths in the assembly,
+pmx_ow_set_cell_dt), accepts its ministeps, is cut and regrows by the rule
+of simulate_schedule_ad (simulate_schedule_ad.py:17-80) on its own, and
+stops when it reaches the report time; the union iterates until every
+member has.  simulate_schedule_ad on the ensemble model instead shares one
+step between all members (a cut for one member is taken by all).
 """
 from dataclasses import dataclass, replace
 from typing import List, Optional, Sequence
@@ -61,13 +68,21 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
                                          self.well_member.repeat_interleave(3)])
         self.member_its = np.zeros(nmem, dtype=np.int64)
         self._ok_m = None
+        self._frozen = None
+        perf_member = np.repeat(np.asarray(well_member), [w.cells.size for w in self.wells]) \
+            if self.nw else np.zeros(0, np.int64)
+        self.perf_member = torch.as_tensor(perf_member, dtype=torch.long, device=dev)
 
     def _seg(self, v, op, idx, n):
         out = torch.zeros(n, dtype=v.dtype, device=v.device)
         return out.scatter_reduce(0, idx, v, reduce=op, include_self=False)
 
+    member_dt = None      # per-member step lengths (torch, nmem) while simulate_ensemble runs
+
     def converged(self, dt: float, st):
         f = self.fluid
+        if self.member_dt is not None:
+            dt = self.member_dt
         r = self.residual_device()
         nc, M = self.nc, self.nmem
         rw, ro = r[0:2 * nc:2], r[1:2 * nc:2]
@@ -90,6 +105,8 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
             well_ok &= wr[:, 2].abs() <= self.tol_well * scale
             bad = self._seg((~well_ok).to(torch.float64), "amax", self.well_member, M)
             ok_m &= bad == 0
+        if self._frozen is not None:      # members already at the report time stay put
+            ok_m |= self._frozen
         self.unconverged_members = int((~ok_m).sum())
         self.last_cnv_mb = (float(cnv.max()), float(mb.max()))
         self._ok_m = ok_m
@@ -109,6 +126,191 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
         super().reset()
         if hasattr(self, "member_its"):
             self.member_its[:] = 0
+        self.member_dt = None
+        self._frozen = None
+        _lib.check(self.lib.pmx_ow_set_cell_dt(self.h, _lib.ptr(None)), "pmx_ow_set_cell_dt")
+
+    # -- per-member time stepping ----------------------------------------------
+    def _set_member_dt(self, dtm: np.ndarray):
+        self.member_dt = torch.as_tensor(dtm, dtype=torch.float64, device=self.device)
+        self._dt_cell = self.member_dt[self.cell_member].contiguous()   # kept alive: the library reads it
+        _lib.check(self.lib.pmx_ow_set_cell_dt(self.h, _lib.ptr(self._dt_cell)), "pmx_ow_set_cell_dt")
+
+    def _copy_members(self, dst, src, sel: np.ndarray):
+        """dst[member slices] = src[member slices] for the members in sel."""
+        m = torch.as_tensor(sel, device=self.device)
+        cm, wm = m[self.cell_member], m[self.well_member]
+        dst.p[cm] = src.p[cm]
+        dst.sw[cm] = src.sw[cm]
+        if self.nw:
+            dst.qws[wm] = src.qws[wm]
+            dst.qos[wm] = src.qos[wm]
+            dst.bhp[wm] = src.bhp[wm]
+
+    def _update_cdp(self, st0, sel: np.ndarray):
+        """Well heads from state0 for the members in sel (once per ministep,
+        as the member alone computes them); the others keep theirs."""
+        if not self.gravity or not self.nw:
+            return None
+        old = self._cdp_dev.clone()
+        self.compute_cdp_per_well(st0)
+        keep = ~torch.as_tensor(sel, device=self.device)[self.perf_member]
+        n = keep.numel()
+        self._cdp_dev[:n][keep] = old[:n][keep]
+        return self._cdp_dev
+
+
+def _member_ranges(model):
+    """Per member: cell, well and perforation ranges in the union (members
+    are laid out one after the other)."""
+    if getattr(model, "_ranges_cache", None) is None:
+        cm = model.cell_member.cpu().numpy()
+        wm = model.well_member.cpu().numpy()
+        pm = model.perf_member.cpu().numpy()
+        def ptr(a):
+            return np.concatenate([[0], np.cumsum(np.bincount(a, minlength=model.nmem))]).astype(np.int64)
+        model._ranges_cache = (ptr(cm), ptr(wm), ptr(pm))
+    return model._ranges_cache
+
+
+def _transfer(src, s_states, dst, d_states, members, back=False):
+    """Copy the member slices of states and of the well heads between a
+    union and a union of a subset: members[j] of src is member j of dst.
+    back copies dst -> src."""
+    sc, sw_, sp = _member_ranges(src)
+    dc, dw, dp = _member_ranges(dst)
+    for j, k in enumerate(members):
+        for a, b in zip(s_states, d_states):
+            x, y = (b, a) if back else (a, b)
+            xs = (dc[j], dc[j + 1]) if back else (sc[k], sc[k + 1])
+            ys = (sc[k], sc[k + 1]) if back else (dc[j], dc[j + 1])
+            y.p[ys[0]:ys[1]] = x.p[xs[0]:xs[1]]
+            y.sw[ys[0]:ys[1]] = x.sw[xs[0]:xs[1]]
+            xw = (dw[j], dw[j + 1]) if back else (sw_[k], sw_[k + 1])
+            yw = (sw_[k], sw_[k + 1]) if back else (dw[j], dw[j + 1])
+            for f in ("qws", "qos", "bhp"):
+                getattr(y, f)[yw[0]:yw[1]] = getattr(x, f)[xw[0]:xw[1]]
+        # the well heads travel too: the next ministep's heads start from them
+        if src.gravity and src.nw:
+            if back:
+                src._cdp_dev[sp[k]:sp[k + 1]] = dst._cdp_dev[dp[j]:dp[j + 1]]
+            else:
+                dst._cdp_dev[dp[j]:dp[j + 1]] = src._cdp_dev[sp[k]:sp[k + 1]]
+    if src.gravity and src.nw:
+        (src if back else dst)._last_cdp = []     # the heads above are the members' current ones
+
+
+def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts, compact, observer, i):
+    """Newton iterations of one union until every member reaches the report
+    time D; per-member arrays are updated in place.  Returns the union
+    iteration count (this level and the levels below)."""
+    M = model.nmem
+    done = np.zeros(M, dtype=bool)
+    tol = 1e-12
+    n_union = 0
+    cells = np.diff(_member_ranges(model)[0])
+    cdp = model._cdp_dev if (model.gravity and model.nw) else None
+    model._set_member_dt(dtm)      # members may arrive in the middle of a ministep
+    while not done.all():
+        act = ~done
+        if compact and cells[act].sum() < compact * cells.sum() and act.sum() < M:
+            # the members still iterating move to a union of their own
+            members = np.flatnonzero(act)
+            sub, _, _ = build_ensemble([model.cases[k] for k in members], model.linear_solver, model.device,
+                                       segmented=model.segmented)
+            for attr in ("max_iterations", "tol_cnv", "tol_mb", "tol_well", "ds_max", "dp_max_rel"):
+                setattr(sub, attr, getattr(model, attr))
+            s0s = sub.device_state(sub.initial_state(model.cases[0].p0, model.cases[0].sw0))
+            s1s = s0s.copy()
+            _transfer(model, (st0, st), sub, (s0s, s1s), members)
+            sl = [a[members].copy() for a in (t, dtm, cuts, it, its_step, restart)]
+            n_union += _run_level(sub, s0s, s1s, D, *sl, max_cuts, compact, observer, i)
+            _transfer(model, (st,), sub, (s1s,), members, back=True)
+            for a, b in zip((t, dtm, cuts, it, its_step), sl[:5]):
+                a[members] = b
+            model.stats["newton"] += sub.stats["newton"]
+            del sub
+            break
+        if restart.any():
+            cdp = model._update_cdp(st0, restart)
+            model._set_member_dt(dtm)
+            restart[:] = False
+        model._frozen = torch.as_tensor(done, device=model.device)
+        model.assemble_device(st0, st, float(D), cdp)
+        _, pmean = model.converged(float(D), st)
+        ok = model._ok_m.cpu().numpy()
+        for m in np.flatnonzero(ok & ~done):     # accept the ministep
+            t[m] += dtm[m]
+            its_step[m] += it[m]
+            if t[m] >= D * (1.0 - tol):
+                done[m] = True
+                continue
+            dtm[m] = min(2.0 * dtm[m], D - t[m]) if cuts[m] else min(dtm[m], D - t[m])
+            it[m] = 0
+            restart[m] = True
+        failed = ~ok & ~done & (it >= model.max_iterations)
+        for m in np.flatnonzero(failed):          # cut from the ministep's start
+            cuts[m] += 1
+            if cuts[m] > max_cuts:
+                raise RuntimeError(f"step {i}: member {m}: Newton failed after {max_cuts} timestep cuts")
+            its_step[m] += it[m]
+            it[m] = 0
+            dtm[m] /= 2.0
+            restart[m] = True
+            if observer:
+                observer("timestep_cut", {"report_step": i, "member": int(m), "dt": float(dtm[m])})
+        if restart.any():
+            accepted = restart & ~failed
+            if accepted.any():
+                model._copy_members(st0, st, accepted)
+            if failed.any():
+                model._copy_members(st, st0, failed)
+            continue
+        if done.all():
+            break
+        # one Newton update of the members still iterating (the others are frozen)
+        model.stats["newton"] += 1
+        n_union += 1
+        model.solve_and_update(st, pmean)
+        it[~ok & ~done] += 1
+    model.member_dt = None
+    model._frozen = None
+    _lib.check(model.lib.pmx_ow_set_cell_dt(model.h, _lib.ptr(None)), "pmx_ow_set_cell_dt")
+    return n_union
+
+
+def simulate_ensemble(model: "EnsembleModel", state0, timesteps: Sequence[float], max_cuts: int = 6,
+                      observer=None, keep_device: bool = False, compact: float = 0.3):
+    """Report steps with per-member time stepping.  Each Newton iteration is
+    one union assembly and one (segmented) solve over the members still
+    iterating; a member that converges accepts its ministep and starts the
+    next one from there (or stops at the report time); a member that reaches
+    max_iterations is cut to half its step from its ministep's start.  When
+    the members still iterating hold fewer than `compact` of the union's
+    cells, they continue in a union of their own (0 disables), so a few slow
+    members do not keep every iteration at the full ensemble's cost.
+    Returns (states per report step, union Newton iterations per report
+    step, Newton iterations per report step and member)."""
+    M = model.nmem
+    state = model.device_state(state0)
+    states, union_its, member_its = [], [], []
+    for i, D in enumerate(np.atleast_1d(np.asarray(timesteps, dtype=float))):
+        if observer:
+            observer("report_step_started", {"report_step": i, "dt": float(D)})
+        st0, st = state.copy(), state.copy()
+        t, dtm = np.zeros(M), np.full(M, D)
+        cuts, it, its_step = (np.zeros(M, dtype=np.int64) for _ in range(3))
+        restart = np.ones(M, dtype=bool)
+        n_union = _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts, compact, observer,
+                             i)
+        state = st
+        union_its.append(n_union)
+        member_its.append(its_step)
+        if observer:
+            observer("step_accepted", {"report_step": i, "dt": float(D), "iterations": int(n_union),
+                                       "member_cuts": int(cuts.sum())})
+        states.append(state.copy() if keep_device else state.to_host())
+    return states, np.array(union_its), np.array(member_its)
 
 
 def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, device=None,
@@ -160,6 +362,8 @@ def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, d
                                     dtype=np.int32)
         _lib.check(model.lib.pmx_ow_set_segments(model.h, len(cases), _lib.ptr(cptr), _lib.ptr(wptr)),
                    "pmx_ow_set_segments")
+    model.cases = list(cases)
+    model.segmented = segmented
     st = model.initial_state(c0.p0, c0.sw0)
     for c, (a, b) in zip(cases, ranges):
         if c.p_init is not None:

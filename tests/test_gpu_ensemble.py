@@ -52,3 +52,38 @@ def test_member_convergence_is_per_member():
     model.assemble_device(d0, d, dt, model._cdp_dev)
     ok, _ = model.converged(dt, d)
     assert not ok and model.unconverged_members == 1
+
+
+@pytest.mark.parametrize("compact", [0.0, 0.9])
+def test_per_member_time_stepping_matches_members_run_alone(compact):
+    """simulate_ensemble: every member takes the ministeps, cuts and Newton
+    iterations it takes alone under simulate_schedule_ad (a Newton cap of 8
+    makes every member cut its first 40-day step, each a different number of
+    times), and ends in the same state; also when the members still
+    iterating are moved into smaller unions as the others finish
+    (compact = 0.9: as soon as one member is done)."""
+    from pymrst_amd.ensemble import build_ensemble, simulate_ensemble
+    from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
+    from pymrst_amd.workloads import build_model, field_waterflood
+    cases = [field_waterflood(dims=d, seed=s, steps=2, step_days=40.0, inj_rate=r, sigma=sg)
+             for d, s, r, sg in (((7, 6, 3), 5, 4e-4, 0.7), ((5, 9, 4), 6, 6e-4, 1.8), ((8, 4, 2), 7, 1e-4, 1.1),
+                                 ((6, 6, 3), 8, 8e-4, 1.5))]
+    maxit = 8
+    model, s0, ranges = build_ensemble(cases, CPRSolver(rtol=1e-12))
+    model.max_iterations = maxit
+    states, union_its, member_its = simulate_ensemble(model, s0, cases[0].dts, compact=compact)
+    cut_somewhere = False
+    for k, (c, (a, b)) in enumerate(zip(cases, ranges)):
+        m, st0 = build_model(c, CPRSolver(rtol=1e-12))
+        m.max_iterations = maxit
+        events = []
+        st, its = simulate_schedule_ad(st0, m, c.dts, observer=lambda e, p: events.append(e))
+        cut_somewhere |= "timestep_cut" in events
+        assert member_its[:, k].tolist() == its.tolist(), (k, member_its[:, k], its)
+        for step in range(len(c.dts)):
+            p_alone = st[step].pressure
+            np.testing.assert_allclose(states[step].pressure[a:b], p_alone, rtol=1e-8)
+            np.testing.assert_allclose(states[step].sW[a:b], st[step].sW, rtol=0, atol=1e-8)
+    assert cut_somewhere
+    # the union iterates as long as its slowest member, not the sum of all members' cuts
+    assert (union_its <= member_its.sum(axis=1)).all()
