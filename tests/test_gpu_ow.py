@@ -237,3 +237,29 @@ def test_nonlinear_solver_drives_the_oil_water_model():
     for a, b in zip(states, ref_states):
         np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
         np.testing.assert_allclose(a.sW, b.sW, rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_cpr_solve_on_a_level_large_enough_for_the_ell_smoother(prec):
+    """A 66x64x64 field case (270,336 pressure rows: the AMG's first level
+    is smoothed from its sliced-ELL copy): one Newton solve of J dx = -r
+    reaches the requested true residual, checked with the C-ABI product."""
+    import ctypes
+    from pymrst_amd import _lib
+    from pymrst_amd.ow import CPRSolver
+    from pymrst_amd.workloads import build_model, field_waterflood
+    case = field_waterflood(dims=(66, 64, 64), steps=1)
+    md, s0 = build_model(case, CPRSolver(rtol=1e-10, precision=prec))
+    ds = md.device_state(s0)
+    cdp = md.compute_cdp_per_well(ds)
+    md.assemble_device(ds, ds.copy(), float(case.dts[0]), cdp)
+    ok, pmean = md.converged(float(case.dts[0]), ds)
+    r = md.residual_device()
+    info = _lib.SolveInfo()
+    rc = md.lib.pmx_ow_solve(md.h, ctypes.byref(md.linear_solver.options()), float(pmean), _lib.ptr(md._dx),
+                             ctypes.byref(info))
+    assert rc == 0 and info.iterations < 200
+    y = torch.empty_like(md._dx)
+    _lib.check(md.lib.pmx_ow_matvec(md.h, _lib.ptr(md._dx), _lib.ptr(y)))
+    res = torch.linalg.norm(y + r) / torch.linalg.norm(r)
+    assert float(res) <= 1e-10 * 1.01, float(res)
