@@ -442,10 +442,15 @@ __global__ void k_multiaxpy_scale(size_t n, int nv, const double* __restrict__ V
 // partial[blk][nv..2nv)  = V_d . w
 // partial[blk][2nv..+3)  = a . a, a . w, w . w
 // Each basis vector is read once.
+// E elements per thread: a block reduces each dot over kRedThreads * E
+// elements, so the cross-lane sums and block barriers, one set per group of
+// kDot2Group basis vectors, are spread over E loads per thread and vector.
 constexpr int kDot2Group = kRedGroup / 2;
-// Tiles are uniform (kRedTile, the last one short) when tstart is null, else
-// tile b covers [tstart[b], tstart[b] + tlen[b]) (segments: no tile straddles
-// two members).
+constexpr int kDot2Elems = 8;      // elements per thread of the single-system solve (1% faster than 4, 16 slower)
+// Tiles are uniform (kRedThreads * E, the last one short) when tstart is
+// null, else tile b covers [tstart[b], tstart[b] + tlen[b]) (segments: no
+// tile straddles two members; tlen <= kRedThreads * E).
+template <int E>
 __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, const double* __restrict__ V,
                                                            size_t ld, const double* __restrict__ a,
                                                            const double* __restrict__ w,
@@ -454,14 +459,15 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, con
                                                            const int* __restrict__ tlen) {
   __shared__ double sh[kRedThreads / 64][kRedGroup];
   const int nd = 2 * nv + 3;
-  const size_t t0 = tstart ? (size_t)tstart[blockIdx.x] : (size_t)blockIdx.x * kRedTile;
-  const size_t len = tstart ? (size_t)tlen[blockIdx.x] : (n - t0 < (size_t)kRedTile ? n - t0 : (size_t)kRedTile);
+  constexpr size_t tile = (size_t)kRedThreads * E;
+  const size_t t0 = tstart ? (size_t)tstart[blockIdx.x] : (size_t)blockIdx.x * tile;
+  const size_t len = tstart ? (size_t)tlen[blockIdx.x] : (n - t0 < tile ? n - t0 : tile);
   const size_t base = t0 + threadIdx.x;
   const size_t end = t0 + len;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  double av[kRedElems], wv[kRedElems];
+  double av[E], wv[E];
 #pragma unroll
-  for (int e = 0; e < kRedElems; ++e) {
+  for (int e = 0; e < E; ++e) {
     const size_t i = base + (size_t)e * kRedThreads;
     av[e] = i < end ? a[i] : 0.0;
     wv[e] = i < end ? w[i] : 0.0;
@@ -474,7 +480,7 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, con
       double sa = 0.0, sw = 0.0;
       if (d < nv) {
 #pragma unroll
-        for (int e = 0; e < kRedElems; ++e) {
+        for (int e = 0; e < E; ++e) {
           const size_t i = base + (size_t)e * kRedThreads;
           if (i < end) {
             const double v = V[(size_t)d * ld + i];
@@ -503,7 +509,7 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, con
   }
   double aa = 0.0, aw = 0.0, ww = 0.0;
 #pragma unroll
-  for (int e = 0; e < kRedElems; ++e) {
+  for (int e = 0; e < E; ++e) {
     aa = fma(av[e], av[e], aa);
     aw = fma(av[e], wv[e], aw);
     ww = fma(wv[e], wv[e], ww);
@@ -875,8 +881,19 @@ __global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __
 #pragma unroll
       for (int j = 0; j < B; ++j) Pinv[i][j] = Di[i][j];
     if (k + 1 < L) {
+      // the backward sweep needs only D'_k^{-1} U_k: store the product
+      double W[B][B];
       blk_load<B>(val + (size_t)up[at] * B * B, Uprev);
-      blk_store<B, T>(Up + at * B * B, Uprev);
+#pragma unroll
+      for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+          double v = 0.0;
+#pragma unroll
+          for (int q = 0; q < B; ++q) v += Di[i][q] * Uprev[q][j];
+          W[i][j] = v;
+        }
+      blk_store<B, T>(Up + at * B * B, W);
     }
   }
 }
@@ -986,8 +1003,10 @@ __global__ void k_gather_off(size_t ne, const int* __restrict__ off_q, const dou
 
 // Block-tridiagonal solves of the lines of one colour, one thread per line:
 // forward y_k = D'_k^{-1} (r2_k - L_k y_{k-1}) (y overwrites r2 in place),
-// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}; then z = P xp + x and, when
-// ycell is given, y_cell = x for the colours and sweeps that follow.  A wave
+// backward x_k = y_k - W_k x_{k+1} with W_k = D'_k^{-1} U_k (stored by the
+// factorisation in place of U_k); then, in the last sweep only (write_z),
+// z = P xp + x, and when ycell is given, y_cell = x for the colours and
+// sweeps that follow.  A wave
 // walks the padded length of its longest line (lines are sorted by length,
 // tables are zero past each line's end and every colour fills whole waves),
 // so every load address is known without a data-dependent index and the
@@ -1004,7 +1023,7 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
                                                 const T* __restrict__ Lo, const T* __restrict__ Up,
                                                 T* __restrict__ r2l, const T* __restrict__ xpl,
                                                 const PrecIO* __restrict__ io, T* __restrict__ ycell,
-                                                const double* __restrict__ pcw) {
+                                                const double* __restrict__ pcw, int write_z) {
   static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
   constexpr int BB = B * B;
   double* __restrict__ z = io->z;
@@ -1013,6 +1032,9 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
   for (int a = 0; a < B; ++a) pc[a] = pcw[a];
   const int l = lc0 + blockIdx.x * blockDim.x + threadIdx.x;
   const int Lp = (len[lc0 + blockIdx.x * blockDim.x] + D - 1) / D * D;
+  // a wave of padding lines only has no positions (the prefetches below
+  // assume Lp >= D); with whole line groups per wave it does not occur
+  if (Lp == 0) return;
   T ri[D][BB], rl[D][BB], rr[D][B];
   auto fload = [&](int k, T (&vi)[BB], T (&vl)[BB], T (&vr)[B]) {
     const size_t at = (size_t)k * nl + l;
@@ -1055,22 +1077,19 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
+  T bu[D][BB], by[D][B], bx[D];
   int bc[D];
-  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
+  auto bload = [&](int k, T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
     const size_t at = (size_t)k * nl + l;
 #pragma unroll
-    for (int q = 0; q < BB; ++q) {
-      vi[q] = inv[at * BB + q];
-      vu[q] = Up[at * BB + q];
-    }
+    for (int q = 0; q < BB; ++q) vu[q] = Up[at * BB + q];
 #pragma unroll
     for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
-    vx = xpl[at];
+    vx = write_z ? xpl[at] : (T)0;
     c = cell[at];
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
+  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bu[u], by[u], bx[u], bc[u]);
   double xn[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) xn[a] = 0.0;
@@ -1078,30 +1097,25 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int k = k0 - u;
-      double tq[B];
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double v = 0.0;
-#pragma unroll
-        for (int q = 0; q < B; ++q) v += (double)bu[u][a * B + q] * xn[q];
-        tq[a] = v;
-      }
+      double xk[B];
 #pragma unroll
       for (int a = 0; a < B; ++a) {
         double v = (double)by[u][a];
 #pragma unroll
-        for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
-        xn[a] = v;
+        for (int q = 0; q < B; ++q) v -= (double)bu[u][a * B + q] * xn[q];
+        xk[a] = v;
       }
+#pragma unroll
+      for (int a = 0; a < B; ++a) xn[a] = xk[a];
       const int c = bc[u];
       if (c >= 0) {
 #pragma unroll
         for (int a = 0; a < B; ++a) {
-          z[(size_t)c * B + a] = xn[a] + pc[a] * (double)bx[u];
+          if (write_z) z[(size_t)c * B + a] = xn[a] + pc[a] * (double)bx[u];
           if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
         }
       }
-      bload(max(k - D, 0), bi[u], bu[u], by[u], bx[u], bc[u]);
+      bload(max(k - D, 0), bu[u], by[u], bx[u], bc[u]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1874,7 +1888,8 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
         launch_check();
         hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
                            C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
-                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
+                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev,
+                           sw + 1 == sweeps ? 1 : 0);
         launch_check();
       }
     }
@@ -2012,6 +2027,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   double* hscr = C->hmat.p;
   const int nb = grid_for(n);
   const int rb = red_blocks(n);
+  const int rb2 = (int)((n + (size_t)kRedThreads * kDot2Elems - 1) / ((size_t)kRedThreads * kDot2Elems));
   // ||b||
   const double bnorm = std::sqrt(std::max(0.0, device_dot(b, b, n, C->partial.p, s)));
   PMX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
@@ -2066,8 +2082,8 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       g[col] = cs[col] * g[col];
       return std::fabs(g[col + 1]);
     };
-    auto reduce_to_host = [&](int nd, std::vector<double>& hh) {
-      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, rb, nd, C->partial.p, C->hdev.p);
+    auto reduce_to_host = [&](int nd, std::vector<double>& hh, int nblk) {
+      hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, nblk, nd, C->partial.p, C->hdev.p);
       launch_check();
       hh.assign(nd, 0.0);
       PMX_HIP(hipMemcpyAsync(hh.data(), C->hdev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
@@ -2097,7 +2113,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
       if (final) {
         hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
         launch_check();
-        reduce_to_host(j + 2, hh);
+        reduce_to_host(j + 2, hh, rb);
         double hsq = 0.0;
         for (int i = 0; i <= j; ++i) {
           H[(size_t)i * m + j] = hh[i];
@@ -2112,10 +2128,10 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
         launch_check();
       } else {
         const int nd = 2 * j + 3;
-        hipLaunchKernelGGL(k_multidot2, dim3(rb), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
+        hipLaunchKernelGGL(k_multidot2<kDot2Elems>, dim3(rb2), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
                            C->partial.p, nullptr, nullptr);
         launch_check();
-        reduce_to_host(nd, hh);
+        reduce_to_host(nd, hh, rb2);
         const double* sv = hh.data();
         const double* tv = hh.data() + j;
         const double alpha = hh[2 * j], bp = hh[2 * j + 1];
@@ -2174,7 +2190,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
         // complete column j now: reorthogonalise slot j+1 by itself
         hipLaunchKernelGGL(k_multidot, dim3(rb), dim3(kRedThreads), 0, s, n, j + 1, C->V.p, ld, w, C->partial.p);
         launch_check();
-        reduce_to_host(j + 2, hh);
+        reduce_to_host(j + 2, hh, rb);
         double ss = 0.0;
         for (int i = 0; i <= j; ++i) ss += hh[i] * hh[i];
         const double nu = std::sqrt(std::max(hh[j + 1] - ss, 0.0));
@@ -2606,7 +2622,7 @@ int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, dou
         if (delayed) final = false;
       } else {
         const int nd = 2 * j + 3;
-        hipLaunchKernelGGL(k_multidot2, dim3(P.ntile), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
+        hipLaunchKernelGGL(k_multidot2<kRedElems>, dim3(P.ntile), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
                            F.partial.p, P.tile_start.p, P.tile_len.p);
         launch_check();
         hipLaunchKernelGGL(k_seg_reduce, dim3(ns), dim3(64), 0, s, ns, nd, P.tptr.p, P.tidx.p, F.partial.p,
