@@ -22,6 +22,10 @@ def __getattr__(name):
     if name in ("NonLinearSolver", "StepReport", "SimulationReport"):
         from . import nonlinear
         return getattr(nonlinear, name)
+    if name in ("SimpleTimeStepSelectorAD", "FactorTimeStepSelectorAD", "IterationCountTimeStepSelectorAD",
+                "StateChangeTimeStepSelectorAD", "rampup_timesteps", "rampupTimesteps"):
+        from . import timesteps
+        return getattr(timesteps, name)
     if name in ("ThreePhaseBlackOilModelAD", "BOState", "DeviceBOState", "DeckBlackOilFluid",
                 "init_deck_blackoil_fluid"):
         from . import bo

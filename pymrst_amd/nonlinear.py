@@ -4,7 +4,8 @@ time-step cutting, over the pmx models' device Newton primitives.
 Reference: pymrst/modules/ad_core/simulators/nonlinear_solver.py
 (StepReport :52-89, SimulationReport :92-110, NonLinearSolver.__init__
 :117-160, solve_ministep :239-381, solve_timestep :621-702,
-simulate_schedule :847-890, _iter_timesteps :955-977).
+_solve_adaptive :770-844, simulate_schedule :847-890, _iter_timesteps
+:955-977).  Time-step selectors: `pymrst_amd.timesteps`.
 
 The models (`pymrst_amd.ow.TwoPhaseOilWaterModelAD`,
 `pymrst_amd.bo.ThreePhaseBlackOilModelAD`) provide three device steps in
@@ -34,6 +35,7 @@ class StepReport:
     dt: float
     residuals: List[Dict[str, float]] = field(default_factory=list)
     cuts: int = 0
+    ministeps: List[float] = field(default_factory=list)       # the ministeps a selector took, summing to dt
     linear_solves: List[tuple] = field(default_factory=list)   # (iterations, relative residual) per solve
     wall_seconds: float = 0.0
     assembly_seconds: float = 0.0
@@ -58,15 +60,18 @@ class SimulationReport:
 
 class NonLinearSolver:
     """Newton with time-step cutting (the reference's defaults: 25
-    iterations, 6 cuts).  `observer(event, payload)` receives the
-    reference's event names: report_step_started, ministep_started,
-    nonlinear_iteration_completed, linear_solve_failed, timestep_cut,
-    step_accepted, step_rejected."""
+    iterations, 6 cuts).  With a `timestep_selector` (one of
+    `pymrst_amd.timesteps`) a report step is crossed in the ministeps the
+    selector picks, the last one trimmed to land on the report time.
+    `observer(event, payload)` receives the reference's event names:
+    report_step_started, ministep_started, nonlinear_iteration_completed,
+    linear_solve_failed, timestep_cut, step_accepted, step_rejected."""
 
     def __init__(self, max_iterations: int = 25, max_timestep_cuts: int = 6, verbose: bool = False,
-                 observer=None):
+                 observer=None, timestep_selector=None):
         self.max_iterations = max_iterations
         self.max_timestep_cuts = max_timestep_cuts
+        self.timestep_selector = timestep_selector
         self.verbose = verbose
         self.observer = observer
         self.observer_failures: List[dict] = []
@@ -127,6 +132,8 @@ class NonLinearSolver:
         if control is not None:
             raise NotImplementedError("control periods are not supported by the device driver")
         state0 = model.device_state(state0)
+        if self.timestep_selector is not None:
+            return self._solve_adaptive(model, state0, dt, control)
         self._context.update(attempt_index=0, ministep_index=0)
         st, report = self.solve_ministep(model, state0, state0.copy(), dt)
         if report.converged:
@@ -158,6 +165,51 @@ class NonLinearSolver:
         return st, StepReport(converged=False, iterations=report.iterations, dt=dt, cuts=self.max_timestep_cuts,
                               residuals=list(last_failure.residuals),
                               linear_solves=list(last_failure.linear_solves))
+
+    # -- one report step in selector-chosen ministeps (_solve_adaptive :770-844)
+    def _solve_adaptive(self, model, state0, dt: float, control=None):
+        """Ministeps from the selector (its cut rule after a failure); the
+        last one is trimmed to the report time and, when trimmed below 0.9
+        of the selector's pick, kept out of its history.  A failure at or
+        below dt / 2^max_timestep_cuts ends the report step."""
+        sel = self.timestep_selector
+        sel.new_control_step(control)
+        dt_min = dt / 2.0 ** self.max_timestep_cuts
+        st, state_prev, dt_prev = state0, None, None
+        t_local, dt_try, failed = 0.0, dt, False
+        iters = cuts = attempt = 0
+        taken: List[float] = []
+        residuals, linear = [], []
+        while t_local < dt * (1.0 - 1e-12):
+            pick = sel.cut_timestep if failed else sel.pick_timestep
+            dt_sel = pick(dt_prev, dt_try, model, self, state_prev, st)
+            remaining = dt - t_local
+            is_final = dt_sel >= remaining * (1.0 - 1e-12)
+            dt_try = remaining if is_final else dt_sel
+            self._context.update(attempt_index=attempt, ministep_index=len(taken))
+            st_new, rep = self.solve_ministep(model, st, st.copy(), dt_try)
+            attempt += 1
+            iters += rep.iterations
+            residuals.extend(rep.residuals)
+            linear.extend(rep.linear_solves)
+            if not is_final or dt_try > 0.9 * dt_sel:
+                sel.store_timestep(rep)
+            if rep.converged:
+                state_prev, st = st, st_new
+                dt_prev = dt_try
+                t_local += dt_try
+                taken.append(dt_try)
+                failed = False
+            else:
+                cuts += 1
+                failed = True
+                if dt_try <= dt_min:
+                    if self.verbose:
+                        print(f"  ministep of {dt_try:.3g} s failed at the floor ({dt_min:.3g} s)")
+                    return st, StepReport(converged=False, iterations=iters, dt=dt, cuts=cuts, ministeps=taken,
+                                          residuals=residuals, linear_solves=linear)
+        return st, StepReport(converged=True, iterations=iters, dt=dt, cuts=cuts, ministeps=taken,
+                              residuals=residuals, linear_solves=linear)
 
     # -- a schedule of report steps (simulate_schedule :847-890) ----------------
     def simulate_schedule(self, model, state0, schedule: Sequence[float], after_step_fn=None,

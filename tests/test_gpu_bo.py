@@ -214,3 +214,40 @@ def test_nonlinear_solver_cuts_match_oracle(maxit, days):
         np.testing.assert_array_equal(a.status, b.status)
         np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
         np.testing.assert_allclose(a.s, b.s, rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("selector", ["iterations", "state"])
+def test_selector_driven_schedule_matches_oracle(selector):
+    """NonLinearSolver with a time-step selector (the reference's
+    _solve_adaptive) on the device black-oil model against the oracle's
+    restatement over the oracle model: the same ministeps (bit for bit when
+    they follow from Newton counts; to 1e-9 when the state change sets them),
+    the same Newton counts, the same states."""
+    from pymrst_amd.nonlinear import NonLinearSolver
+    from pymrst_amd.ow import CPRSolver
+    import oracle.timesteps as ot
+    import pymrst_amd.timesteps as pt
+    mo, md = _pair(dims=(6, 5, 3), seed=7, wells="injector1", linear_solver=CPRSolver(rtol=1e-12))
+    st0 = mo.initial_state(200 * og.BARSA, 0.25, 0.0, rs0=45.0)
+    st0.bhp[:] = [200 * og.BARSA, 150 * og.BARSA]
+    dts = np.array([30.0, 90.0]) * og.DAY
+    if selector == "iterations":
+        kw = dict(target_iteration_count=3, max_timestep=40 * og.DAY, first_rampup_step=2 * og.DAY)
+        o, p = ot.IterationCount(**kw), pt.IterationCountTimeStepSelectorAD(**kw)
+    else:
+        kw = dict(target_props=["pressure", "s"], target_change_abs=[20 * og.BARSA, 0.02],
+                  first_rampup_step=2 * og.DAY)
+        o, p = ot.StateChange(**kw), pt.StateChangeTimeStepSelectorAD(**kw)
+    ref_states, ref_its, ref_mini = ot.simulate_schedule_adaptive(mo, st0, dts, o)
+    states, rep = NonLinearSolver(timestep_selector=p).simulate_schedule(md, st0, dts)
+    assert rep.converged and [s.iterations for s in rep.steps] == ref_its.tolist()
+    assert sum(len(m) for m in ref_mini) > 2 * len(dts), "the selector should subdivide the report steps"
+    for r, m in zip(rep.steps, ref_mini):
+        if selector == "iterations":
+            assert r.ministeps == m
+        else:
+            np.testing.assert_allclose(r.ministeps, m, rtol=1e-9)
+    for a, b in zip(states, ref_states):
+        np.testing.assert_array_equal(a.status, b.status)
+        np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
+        np.testing.assert_allclose(a.s, b.s, rtol=0, atol=1e-8)

@@ -263,3 +263,27 @@ def test_cpr_solve_on_a_level_large_enough_for_the_ell_smoother(prec):
     _lib.check(md.lib.pmx_ow_matvec(md.h, _lib.ptr(md._dx), _lib.ptr(y)))
     res = torch.linalg.norm(y + r) / torch.linalg.norm(r)
     assert float(res) <= 1e-10 * 1.01, float(res)
+
+
+def test_selector_driven_schedule_matches_oracle():
+    """NonLinearSolver with the iteration-count selector on the oil-water
+    device model (gravity, a cross-flowing well) against the oracle's
+    selector-driven report steps: identical ministeps and Newton counts,
+    pressure within 1e-8 relative, saturation within 1e-8."""
+    from pymrst_amd.nonlinear import NonLinearSolver
+    from pymrst_amd.ow import CPRSolver
+    import oracle.timesteps as ot
+    import pymrst_amd.timesteps as pt
+    inp = small_gravity_inputs()
+    mo, so = build_oracle(inp)
+    md, sd = build_device(inp, CPRSolver(rtol=1e-12))
+    dts = np.array([20.0, 60.0]) * og.DAY
+    kw = dict(target_iteration_count=4, first_rampup_step=1 * og.DAY)
+    ref_states, ref_its, ref_mini = ot.simulate_schedule_adaptive(mo, so, dts, ot.IterationCount(**kw))
+    states, rep = NonLinearSolver(timestep_selector=pt.IterationCountTimeStepSelectorAD(**kw)).simulate_schedule(
+        md, sd, dts)
+    assert rep.converged and [s.iterations for s in rep.steps] == ref_its.tolist()
+    assert [s.ministeps for s in rep.steps] == ref_mini and sum(len(m) for m in ref_mini) > 2 * len(dts)
+    for a, b in zip(states, ref_states):
+        np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
+        np.testing.assert_allclose(a.sW, b.sW, rtol=0, atol=1e-8)
