@@ -881,19 +881,8 @@ __global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __
 #pragma unroll
       for (int j = 0; j < B; ++j) Pinv[i][j] = Di[i][j];
     if (k + 1 < L) {
-      // the backward sweep needs only D'_k^{-1} U_k: store the product
-      double W[B][B];
       blk_load<B>(val + (size_t)up[at] * B * B, Uprev);
-#pragma unroll
-      for (int i = 0; i < B; ++i)
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-          double v = 0.0;
-#pragma unroll
-          for (int q = 0; q < B; ++q) v += Di[i][q] * Uprev[q][j];
-          W[i][j] = v;
-        }
-      blk_store<B, T>(Up + at * B * B, W);
+      blk_store<B, T>(Up + at * B * B, Uprev);
     }
   }
 }
@@ -1003,10 +992,10 @@ __global__ void k_gather_off(size_t ne, const int* __restrict__ off_q, const dou
 
 // Block-tridiagonal solves of the lines of one colour, one thread per line:
 // forward y_k = D'_k^{-1} (r2_k - L_k y_{k-1}) (y overwrites r2 in place),
-// backward x_k = y_k - W_k x_{k+1} with W_k = D'_k^{-1} U_k (stored by the
-// factorisation in place of U_k); then, in the last sweep only (write_z),
-// z = P xp + x, and when ycell is given, y_cell = x for the colours and
-// sweeps that follow.  A wave
+// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}; then z = P xp + x and, when
+// ycell is given, y_cell = x for the colours and sweeps that follow.  (The
+// same walk with D'^{-1} U premultiplied and z written in the last sweep
+// only reads and writes fewer bytes but measured 7% slower per launch.)  A wave
 // walks the padded length of its longest line (lines are sorted by length,
 // tables are zero past each line's end and every colour fills whole waves),
 // so every load address is known without a data-dependent index and the
@@ -1023,7 +1012,7 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
                                                 const T* __restrict__ Lo, const T* __restrict__ Up,
                                                 T* __restrict__ r2l, const T* __restrict__ xpl,
                                                 const PrecIO* __restrict__ io, T* __restrict__ ycell,
-                                                const double* __restrict__ pcw, int write_z) {
+                                                const double* __restrict__ pcw) {
   static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
   constexpr int BB = B * B;
   double* __restrict__ z = io->z;
@@ -1077,19 +1066,22 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  T bu[D][BB], by[D][B], bx[D];
+  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
   int bc[D];
-  auto bload = [&](int k, T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
+  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
     const size_t at = (size_t)k * nl + l;
 #pragma unroll
-    for (int q = 0; q < BB; ++q) vu[q] = Up[at * BB + q];
+    for (int q = 0; q < BB; ++q) {
+      vi[q] = inv[at * BB + q];
+      vu[q] = Up[at * BB + q];
+    }
 #pragma unroll
     for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
-    vx = write_z ? xpl[at] : (T)0;
+    vx = xpl[at];
     c = cell[at];
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bu[u], by[u], bx[u], bc[u]);
+  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
   double xn[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) xn[a] = 0.0;
@@ -1097,25 +1089,30 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       const int k = k0 - u;
-      double xk[B];
+      double tq[B];
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += (double)bu[u][a * B + q] * xn[q];
+        tq[a] = v;
+      }
 #pragma unroll
       for (int a = 0; a < B; ++a) {
         double v = (double)by[u][a];
 #pragma unroll
-        for (int q = 0; q < B; ++q) v -= (double)bu[u][a * B + q] * xn[q];
-        xk[a] = v;
+        for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
+        xn[a] = v;
       }
-#pragma unroll
-      for (int a = 0; a < B; ++a) xn[a] = xk[a];
       const int c = bc[u];
       if (c >= 0) {
 #pragma unroll
         for (int a = 0; a < B; ++a) {
-          if (write_z) z[(size_t)c * B + a] = xn[a] + pc[a] * (double)bx[u];
+          z[(size_t)c * B + a] = xn[a] + pc[a] * (double)bx[u];
           if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
         }
       }
-      bload(max(k - D, 0), bu[u], by[u], bx[u], bc[u]);
+      bload(max(k - D, 0), bi[u], bu[u], by[u], bx[u], bc[u]);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1888,8 +1885,7 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
         launch_check();
         hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
                            C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
-                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev,
-                           sw + 1 == sweeps ? 1 : 0);
+                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
         launch_check();
       }
     }
