@@ -1247,11 +1247,16 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict
       a[(size_t)k * n + j] *= d;
     }
     __syncthreads();
-    // eliminate column k from the other rows
-    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-      const int i = idx / n, j = idx % n;
-      if (i == k || j == k) continue;
-      a[(size_t)i * n + j] -= a[(size_t)i * n + k] * a[(size_t)k * n + j];
+    // eliminate column k from the other rows: one wave per row, lanes along
+    // the row (column k and row k are only read here)
+    {
+      const int lane = threadIdx.x & 63, nwave = blockDim.x >> 6;
+      for (int i = threadIdx.x >> 6; i < n; i += nwave) {
+        if (i == k) continue;
+        const double aik = a[(size_t)i * n + k];
+        for (int j = lane; j < n; j += 64)
+          if (j != k) a[(size_t)i * n + j] -= aik * a[(size_t)k * n + j];
+      }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1331,11 +1336,15 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(int nlev, const
     __syncthreads();
   }
   {
+    // one wave per row of the dense inverse, its lanes along the row
+    // (coalesced; a row-per-thread walk is one dependent load per entry)
     const CoarseLevel<T> a = L[nlev - 1];
-    for (int i = tid; i < a.n; i += blockDim.x) {
+    const int lane = tid & 63, nwave = blockDim.x >> 6;
+    for (int i = tid >> 6; i < a.n; i += nwave) {
       double sI = 0.0;
-      for (int j = 0; j < a.n; ++j) sI += dense[(size_t)i * a.n + j] * (double)a.b[j];
-      a.x[i] = (T)sI;
+      for (int j = lane; j < a.n; j += 64) sI += dense[(size_t)i * a.n + j] * (double)a.b[j];
+      sI = wave_sum(sI, 64);
+      if (lane == 0) a.x[i] = (T)sI;
     }
     if (tid == 0) xs[nlev - 1] = a.x;
     __syncthreads();
