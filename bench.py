@@ -44,7 +44,8 @@ def run_form(case, precision, steps, warmup, rtol):
     import torch
     from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
     from pymrst_amd.workloads import build_model
-    solver = CPRSolver(rtol=rtol, precision=precision, verbose=1)
+    # no per-solve log lines inside the timed region (the observer reports each report step)
+    solver = CPRSolver(rtol=rtol, precision=precision, verbose=0)
     t_setup = time.perf_counter()
     model, s0 = build_model(case, solver)
     print(f"[bench] FP{precision}: model set up in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr,
@@ -250,7 +251,7 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     dts = cases[0].dts
     # a Newton path that diverges for one member makes its linear solve
     # fail; 20 restart cycles let that fail fast into a time-step cut
-    solver = CPRSolver(precision=32, verbose=1, maxiter=20)
+    solver = CPRSolver(precision=32, verbose=0, maxiter=20)
     t0 = time.perf_counter()
     model, s0, ranges = build_ensemble(cases, solver)
     model.max_iterations = 40
