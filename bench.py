@@ -40,10 +40,11 @@ def spmv_bytes(nnzb: int, nc: int) -> float:
     return nnzb * 36.0 + 4.0 * (nc + 1) + 16.0 * nc * 2
 
 
-# CPR settings of the oil-water field runs: five line Gauss-Seidel sweeps and
-# undamped Jacobi smoothing in the AMG (the CPRSolver defaults, 4 and 0.67,
-# suit the black-oil case; on this case 5 / 1.0 take 1,480 FGMRES iterations
-# instead of 1,796 with the same Newton iterations)
+# CPR settings of the oil-water runs (field case and ensemble, both forms of
+# each comparison): five line Gauss-Seidel sweeps instead of the default four
+# (the black-oil case runs best at four); with undamped Jacobi smoothing the
+# field case takes 1,480 FGMRES iterations instead of 1,796 at the earlier
+# 4 sweeps / omega 0.67, with the same Newton iterations
 OW_CPR = {"local_sweeps": 5, "jacobi_omega": 1.0}
 
 
@@ -258,7 +259,7 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     dts = cases[0].dts
     # a Newton path that diverges for one member makes its linear solve
     # fail; 20 restart cycles let that fail fast into a time-step cut
-    solver = CPRSolver(precision=32, verbose=0, maxiter=20)
+    solver = CPRSolver(precision=32, verbose=0, maxiter=20, **OW_CPR)
     t0 = time.perf_counter()
     model, s0, ranges = build_ensemble(cases, solver)
     model.max_iterations = 40
@@ -294,7 +295,7 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     tot_cells_its, tot_wall, ncl, loop_its = 0, 0.0, 0, []
     for k, c in enumerate(cases[:loop_sample]):
         print(f"[bench] ensemble: looped member {k} ({c.num_cells} cells)", file=sys.stderr, flush=True)
-        sv = CPRSolver(precision=64, maxiter=20)
+        sv = CPRSolver(precision=64, maxiter=20, **OW_CPR)
         m, st0 = build_model(c, sv)
         m.max_iterations = 40
         if warmup:
@@ -320,7 +321,7 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
                         f"sigma(ln k)~U(0.5,2), 5 x 30 d",
             "metric_note": "cell-Newton-iterations of each member (each with its own time steps; members "
                            "that have converged or reached the report time are frozen) per wall second",
-            "members": members, "cells": ncells,
+            "members": members, "cells": ncells, "cpr": dict(OW_CPR),
             "batched_fp32_precond": batched, "looped_plain_fp64_sample": looped,
             "speedup_batched_vs_looped_fp64": round(batched["value"] / looped["value"], 2),
             "time_to_solution_ratio_looped_vs_batched": round(looped["us_per_member_cell"] /

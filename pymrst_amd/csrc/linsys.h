@@ -64,7 +64,7 @@ struct LinearSolverOptions {
   int precision = 64;        // 64: FP64 preconditioner; 32: FP32 preconditioner
   int use_pressure_stage = 1;
   int pre_smooth = 2, post_smooth = 2;
-  double jacobi_omega = 0.67;
+  double jacobi_omega = 1.0;  // AMG Jacobi damping (1.0 measured best on the oil-water and black-oil cases)
   double coarse_alpha = 1.5; // coarse-correction scaling (over-correction for unsmoothed aggregation)
   int amg_cycles = 1;
   double theta = 0.5;

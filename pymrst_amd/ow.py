@@ -103,7 +103,7 @@ class CPRSolver:
     use_pressure_stage: bool = True
     pre_smooth: int = 2
     post_smooth: int = 2
-    jacobi_omega: float = 0.67
+    jacobi_omega: float = 1.0
     coarse_alpha: float = 1.5
     theta: float = 0.5
     coarse_size: int = 128
