@@ -328,6 +328,29 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
                                                               batched["us_per_member_cell"], 2)}
 
 
+def stream_copy(reps: int = 20, mib: int = 1024):
+    """STREAM-style copy bandwidth of this GPU: `reps` device-to-device
+    copies of a `mib` MiB FP64 buffer (four times the 256 MB Infinity
+    Cache), timed with events on torch's stream; read and write bytes both
+    counted."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * n * 8
+    del a, b
+    return {"GBs": round(moved / ms / 1e6, 1), "bytes_per_copy": moved, "avg_ms": round(ms, 4),
+            "buffer_MiB": mib, "reps": reps}
+
+
 def traffic_probe(case, reps):
     """A short program for hardware-counter runs: the FP64 BCSR SpMV of the
     assembled Jacobian `reps` times, then a plain 512 MiB device copy `reps`
@@ -449,6 +472,7 @@ def main():
                        "cells": nc, "report_steps": steps, "cpr": dict(OW_CPR)},
             "roofline": roof, "cpu_baseline": cpu}
     line.update(extra)
+    line["stream_copy"] = stream_copy()
     if not args.no_blackoil:
         line["blackoil"] = blackoil_leg(cpu=not args.no_cpu_baseline)
     if not args.no_adjoint:
