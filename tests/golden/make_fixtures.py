@@ -13,6 +13,9 @@ Sources (relative to the reference checkout):
   (case in pymrst/modules/ad_core/examples/paired_heterogeneous_waterflood.py)
 * examples/validation/ref_p4_ad.mat - MRST compressible single-phase AD run
 * examples/validation/ref_ad_blackoil_phase_flux.mat - per-face black-oil flux
+* examples/validation/ref_ad_core_timesteps.mat - MRST's time-step selectors
+  driven through eight scripted ministeps, and rampupTimesteps (script
+  examples/validation/dump_ad_core_timesteps_reference.m)
 
 Run:  python tests/golden/make_fixtures.py [/path/to/reference]
 """
@@ -39,6 +42,7 @@ def main(ref_root="/root/reference"):
     np.savez_compressed(HERE / "p4_ad.npz", **_mat(val / "ref_p4_ad.mat"))
     np.savez_compressed(HERE / "bo_phase_flux.npz",
                         **_mat(val / "ref_ad_blackoil_phase_flux.mat"))
+    np.savez_compressed(HERE / "ad_core_timesteps.npz", **_mat(val / "ref_ad_core_timesteps.mat"))
     pw = np.load(ref / "paired_waterflood_pymrst_results.npz", allow_pickle=False)
     np.savez_compressed(HERE / "paired_waterflood.npz",
                         **{k: np.asarray(pw[k]) for k in pw.files})

@@ -1,6 +1,8 @@
 """Time-step selectors and the selector-driven report step (CPU).
 
-The oracle's selectors are pinned by the known answers of the reference's
+The oracle's selectors are pinned by MRST's own run of the selectors
+(examples/validation/ref_ad_core_timesteps.mat, converted to
+tests/golden/ad_core_timesteps.npz) and the known answers of the reference's
 own tests (pymrst/modules/ad_core/timesteps/timestep_selectors.py:408-516,
 simple_time_step_selector.py:160-200, iteration_count_time_step_selector.py
 :76-106, rampup_timesteps.py:40-57, pymrst/utils/matlab_sum.py:47-57); the
@@ -110,6 +112,60 @@ def test_oracle_rampup_and_matlab_sum():
     d = ot.rampup_timesteps(YEAR, 32 * DAY, n=5)
     assert abs(d[0] - d[1]) < 1e-9 * DAY and np.allclose(d[2:6] / d[1:5], 2.0)
     assert abs(d[0] - 32 * DAY / 2 ** 5) < 1e-9 * DAY
+
+
+def _mrst_script(selectors, dts_ref, its, conv, ctrl, sw):
+    """The scripted sequence of dump_ad_core_timesteps_reference.m: control
+    changes (and a repeated control every third step), picks, a cut after the
+    failed step, step reports; every chosen step compared."""
+    out = []
+    for sel in selectors:
+        dt_prev, state_prev, dt, dts = None, None, 30 * DAY, []
+        for i in range(its.size):
+            if i == 0 or ctrl[i] != ctrl[i - 1] or (i + 1) % 3 == 0:
+                sel.new_control_step({"controlId": int(ctrl[i])})
+            state = SimpleNamespace(s=np.column_stack([np.full(3, sw[i]), np.full(3, 1 - sw[i])]),
+                                    pressure=np.full(3, 1e7))
+            if i > 0 and not conv[i - 1]:
+                dt = sel.cut_timestep(dt_prev, dt, None, SOLVER, state_prev, state)
+            else:
+                dt = sel.pick_timestep(dt_prev, dt, None, SOLVER, state_prev, state)
+            dts.append(dt)
+            sel.store_timestep(rep(int(its[i]), dt, ok=bool(conv[i])))
+            if conv[i]:
+                dt_prev, state_prev = dt, state
+        out.append(dts)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_selectors_and_rampup_match_mrst_fixture(impl):
+    """MRST 2024a's Simple (ramp-up, hard limit), IterationCount, StateChange
+    (saturation, iteration target infinite) and Factor (reset on control
+    change) selectors through eight scripted ministeps, and rampupTimesteps in
+    three cases (ref_ad_core_timesteps.mat): every step to 1e-14."""
+    from cases import golden
+    ref = golden("ad_core_timesteps.npz")
+    mod = ot if impl == "oracle" else pt
+    if impl == "oracle":
+        sels = [ot.Simple(first_rampup_step_relative=0.1, first_rampup_step=2 * DAY, max_timestep=40 * DAY),
+                ot.IterationCount(target_iteration_count=8, first_rampup_step_relative=0.1),
+                ot.StateChange(target_props=["s"], target_change_abs=[0.2], target_iteration_count=math.inf),
+                ot.Factor(max_relative_adjustment=1.5, reset_on_controls_changed=True)]
+    else:
+        sels = [pt.SimpleTimeStepSelectorAD(first_rampup_step_relative=0.1, first_rampup_step=2 * DAY,
+                                            max_timestep=40 * DAY),
+                pt.IterationCountTimeStepSelectorAD(target_iteration_count=8, first_rampup_step_relative=0.1),
+                pt.StateChangeTimeStepSelectorAD(target_props=["s"], target_change_abs=[0.2],
+                                                 target_iteration_count=math.inf),
+                pt.FactorTimeStepSelectorAD(max_relative_adjustment=1.5, reset_on_controls_changed=True)]
+    dts = _mrst_script(sels, ref["dts"], ref["its"], ref["conv"].astype(bool), ref["ctrl"], ref["sw"])
+    np.testing.assert_allclose(dts, ref["dts"], rtol=1e-14, atol=0)
+    for args, key in (((100 * DAY, 10 * DAY), "ramp1"), ((3 * DAY, 10 * DAY, 4), "ramp2"),
+                      ((105 * DAY, 10 * DAY, 3), "ramp3")):
+        d = mod.rampup_timesteps(*args)
+        assert d.shape == np.atleast_1d(ref[key]).shape
+        np.testing.assert_allclose(d, np.atleast_1d(ref[key]), rtol=1e-14, atol=0)
 
 
 # -- product against the oracle ------------------------------------------------
