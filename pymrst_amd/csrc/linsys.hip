@@ -944,6 +944,64 @@ __global__ __launch_bounds__(256) void k_line_r1(int nc, const int* __restrict__
   }
 }
 
+// k_line_r1 on the finest level's sliced-ELL layout (the pressure system
+// shares the Jacobian's block pattern, so the level's slot map places the
+// pressure column Jp too): one thread per row, every load of a wave
+// coalesced; padding slots hold zero and the row's own column.
+template <class T, int B>
+__global__ void k_to_sell_vec(int nnz, const int* __restrict__ map, const T* __restrict__ v, T* __restrict__ sv) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+#pragma unroll
+  for (int a = 0; a < B; ++a) sv[(size_t)map[k] * B + a] = v[(size_t)k * B + a];
+}
+
+template <int B, int X, class T>
+__global__ __launch_bounds__(256) void k_line_r1_sell(int nc, const int* __restrict__ sl_ptr,
+                                                      const int* __restrict__ sl_len, const int* __restrict__ sl_col,
+                                                      const T* __restrict__ Jps, const T* __restrict__ xp,
+                                                      const PrecIO* __restrict__ io, const int* __restrict__ pc_of,
+                                                      const int* __restrict__ pc_ptr,
+                                                      const int* __restrict__ pc_perf,
+                                                      const int* __restrict__ perf_well,
+                                                      const double* __restrict__ Um, const double* __restrict__ hp,
+                                                      const int* __restrict__ pos, T* __restrict__ r1l,
+                                                      T* __restrict__ xpl) {
+  const double* __restrict__ r = io->r;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nc) return;
+  const int sl = row >> 6;
+  const int base = sl_ptr[sl] + (row & 63), len = sl_len[sl];
+  double acc[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) acc[a] = 0.0;
+#pragma unroll 4
+  for (int k = 0; k < len; ++k) {
+    const size_t q = (size_t)base + 64 * (size_t)k;
+    const double xj = (double)xp[sl_col[q]];
+#pragma unroll
+    for (int a = 0; a < B; ++a) acc[a] = fma((double)Jps[q * B + a], xj, acc[a]);
+  }
+  double r1[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) r1[a] = r[(size_t)row * B + a] - acc[a];
+  if (pc_of != nullptr) {
+    const int u = pc_of[row];
+    if (u >= 0)
+      for (int m = pc_ptr[u]; m < pc_ptr[u + 1]; ++m) {
+        const int j = pc_perf[m], w = perf_well[j];
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+          for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
+      }
+  }
+  const size_t at = (size_t)pos[row];
+#pragma unroll
+  for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
+  xpl[at] = xp[row];
+}
+
 // Right-hand sides of the lines of colour `color` (one thread per line
 // position): r2 = r1 - sum_h Aoff_h y_nbr_h over the couplings off the line;
 // in the first sweep only neighbours of earlier colours have a value yet.
@@ -1404,8 +1462,8 @@ class CprSolver {
   bool built = false;
   int coarse_n = 0;
   DArr<double> dense;                 // coarsest: inverse (n x n)
-  DArr<double> inv64, tail_inv, Jp64;
-  DArr<float> inv32, Jp32;
+  DArr<double> inv64, tail_inv, Jp64, Jps64;   // Jps: Jp in the finest level's ELL slot order
+  DArr<float> inv32, Jp32, Jps32;
   DArr<double> hp, hmat;              // per-well scratch
   // Krylov workspace
   DArr<double> V, Z, w, partial, hdev, ydev, r;
@@ -1590,6 +1648,7 @@ struct Lv<double> {
   static DArr<double>& b(AmgLevel& l) { return l.b64; }
   static DArr<double>& r(AmgLevel& l) { return l.r64; }
   static DArr<double>& Jp(CprSolver& c) { return c.Jp64; }
+  static DArr<double>& Jps(CprSolver& c) { return c.Jps64; }
   static DArr<double>& inv(CprSolver& c) { return c.inv64; }
   static DArr<double>& lf_inv(CprSolver& c) { return c.lf_inv64; }
   static DArr<double>& lf_lo(CprSolver& c) { return c.lf_lo64; }
@@ -1612,6 +1671,7 @@ struct Lv<float> {
   static DArr<float>& b(AmgLevel& l) { return l.b32; }
   static DArr<float>& r(AmgLevel& l) { return l.r32; }
   static DArr<float>& Jp(CprSolver& c) { return c.Jp32; }
+  static DArr<float>& Jps(CprSolver& c) { return c.Jps32; }
   static DArr<float>& inv(CprSolver& c) { return c.inv32; }
   static DArr<float>& lf_inv(CprSolver& c) { return c.lf_inv32; }
   static DArr<float>& lf_lo(CprSolver& c) { return c.lf_lo32; }
@@ -1692,6 +1752,14 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
       hipLaunchKernelGGL((k_to_sell<T>), dim3(grid_for(a.nnz)), dim3(256), 0, s, a.nnz, a.sl_map.p, V::v(a).p,
                          V::sv(a).p);
       launch_check();
+      if (l == 0 && o.local_stage == 1 && C->nlines > 0) {
+        PMX_CHECK(a.nnz == S.nnzb && a.n == S.nc, "the finest level shares the Jacobian's block pattern");
+        V::Jps(*C).alloc(a.sell_slots * B);
+        V::Jps(*C).zero(s);
+        hipLaunchKernelGGL((k_to_sell_vec<T, B>), dim3(grid_for(a.nnz)), dim3(256), 0, s, a.nnz, a.sl_map.p,
+                           V::Jp(*C).p, V::Jps(*C).p);
+        launch_check();
+      }
     }
     if (l + 1 < C->L.size()) {
       AmgLevel& c = *C->L[l + 1];
@@ -1878,9 +1946,15 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
     const int sweeps = std::max(1, o.local_sweeps);
     constexpr int D = line_depth<B, T>();
     const size_t npos = (size_t)C->maxlen * C->nlines;
-    hipLaunchKernelGGL((k_line_r1<B, X, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                       S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, mix ? S.pc_of.p : nullptr, S.pc_ptr.p,
-                       S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, C->ln_pos.p, V::r1l(*C).p, V::xpl(*C).p);
+    if (f.nslice > 0)
+      hipLaunchKernelGGL((k_line_r1_sell<B, X, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, f.sl_ptr.p,
+                         f.sl_len.p, f.sl_col.p, V::Jps(*C).p, V::x(f).p, C->io, mix ? S.pc_of.p : nullptr,
+                         S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, C->ln_pos.p, V::r1l(*C).p,
+                         V::xpl(*C).p);
+    else
+      hipLaunchKernelGGL((k_line_r1<B, X, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                         S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, mix ? S.pc_of.p : nullptr, S.pc_ptr.p,
+                         S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, C->ln_pos.p, V::r1l(*C).p, V::xpl(*C).p);
     launch_check();
     for (int sw = 0; sw < sweeps; ++sw) {
       for (int c = 0; c < C->ncolor; ++c) {
