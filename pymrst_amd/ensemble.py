@@ -217,7 +217,7 @@ def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts,
             # the members still iterating move to a union of their own
             members = np.flatnonzero(act)
             sub, _, _ = build_ensemble([model.cases[k] for k in members], model.linear_solver, model.device,
-                                       segmented=model.segmented)
+                                       segmented=model.segmented, pieces=[model.pieces[k] for k in members])
             for attr in ("max_iterations", "tol_cnv", "tol_mb", "tol_well", "ds_max", "dp_max_rel"):
                 setattr(sub, attr, getattr(model, attr))
             s0s = sub.device_state(sub.initial_state(model.cases[0].p0, model.cases[0].sw0))
@@ -313,40 +313,68 @@ def simulate_ensemble(model: "EnsembleModel", state0, timesteps: Sequence[float]
     return states, np.array(union_its), np.array(member_its)
 
 
+@dataclass
+class MemberPieces:
+    """A member's static data in its own cell numbering: connections,
+    transmissibilities, geometry, rock, wells and lines."""
+    nc: int
+    N: np.ndarray
+    T: np.ndarray
+    centroids: np.ndarray
+    volumes: np.ndarray
+    perm: np.ndarray
+    poro: np.ndarray
+    wells: list
+    line_ptr: np.ndarray
+    line_cells: np.ndarray
+
+
+def member_pieces(c) -> MemberPieces:
+    g = cart_grid(c.dims, c.phys)
+    rock = make_rock(g, c.perm, c.poro)
+    N, T, _, _ = tpfa_connections(g, compute_trans(g, rock))
+    grav = np.array([0.0, 0.0, c.gravity]) if c.gravity else None
+    mw = None
+    for cells, kind, val, compi in c.wells:
+        mw = add_well(mw, g, rock, cells, kind, val, compi=np.array(compi), sign=1 if kind == "rate" else -1,
+                      gravity=grav)
+    _, ptr, lc = cartesian_lines(g, N, T)
+    return MemberPieces(g.num_cells, N, T, g.cell_centroids, g.cell_volumes, rock.perm, rock.poro, list(mw or []),
+                        np.asarray(ptr), np.asarray(lc))
+
+
 def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, device=None,
-                   segmented: bool = True) -> tuple:
+                   segmented: bool = True, pieces: Optional[Sequence[MemberPieces]] = None) -> tuple:
     """(EnsembleModel, initial host state, member cell ranges) for a list of
     OWCase members sharing one fluid (the first member's).  With
-    `segmented`, FGMRES keeps one Krylov recurrence per member."""
+    `segmented`, FGMRES keeps one Krylov recurrence per member.  `pieces`
+    (one MemberPieces per case, e.g. a larger union's `model.pieces`) skips
+    recomputing the members' static data."""
     c0 = cases[0]
     fluid = init_simple_adi_fluid(c0.mu, c0.rho, c0.n, c0.c, c0.p_ref, c_r=c0.c_r)
+    if pieces is None:
+        pieces = [member_pieces(c) for c in cases]
+    member_data = list(pieces)
+    if len(member_data) != len(cases):
+        raise ValueError("one MemberPieces per case")
     Ns, Ts, cents, vols, perms, poros, wells, lines_ptr, lines_cells = [], [], [], [], [], [], [], [0], []
     cell_member, well_member, ranges = [], [], []
     off = 0
-    for m, c in enumerate(cases):
-        g = cart_grid(c.dims, c.phys)
-        rock = make_rock(g, c.perm, c.poro)
-        N, T, _, _ = tpfa_connections(g, compute_trans(g, rock))
-        Ns.append(N + off)
-        Ts.append(T)
-        cents.append(g.cell_centroids)
-        vols.append(g.cell_volumes)
-        perms.append(rock.perm)
-        poros.append(rock.poro)
-        grav = np.array([0.0, 0.0, c.gravity]) if c.gravity else None
-        mw = None
-        for cells, kind, val, compi in c.wells:
-            mw = add_well(mw, g, rock, cells, kind, val, compi=np.array(compi), sign=1 if kind == "rate" else -1,
-                          gravity=grav)
-        for w in mw:
+    for m, (c, pc) in enumerate(zip(cases, member_data)):
+        Ns.append(pc.N + off)
+        Ts.append(pc.T)
+        cents.append(pc.centroids)
+        vols.append(pc.volumes)
+        perms.append(pc.perm)
+        poros.append(pc.poro)
+        for w in pc.wells:
             wells.append(replace(w, cells=w.cells + off, name=f"M{m}_{w.name}"))
             well_member.append(m)
-        axis, ptr, lc = cartesian_lines(g, N, T)
-        lines_cells.append(lc + off)
-        lines_ptr.extend((ptr[1:] + lines_ptr[-1]).tolist())
-        cell_member.append(np.full(g.num_cells, m))
-        ranges.append((off, off + g.num_cells))
-        off += g.num_cells
+        lines_cells.append(pc.line_cells + off)
+        lines_ptr.extend((pc.line_ptr[1:] + lines_ptr[-1]).tolist())
+        cell_member.append(np.full(pc.nc, m))
+        ranges.append((off, off + pc.nc))
+        off += pc.nc
     ug = UnionGrid(num_cells=off, cell_centroids=np.vstack(cents), cell_volumes=np.concatenate(vols))
     rock = Rock(perm=np.vstack(perms), poro=np.concatenate(poros))
     model = EnsembleModel(ug, rock, fluid, wells, gravity=c0.gravity, linear_solver=linear_solver, device=device,
@@ -363,6 +391,7 @@ def build_ensemble(cases: Sequence, linear_solver: Optional[CPRSolver] = None, d
         _lib.check(model.lib.pmx_ow_set_segments(model.h, len(cases), _lib.ptr(cptr), _lib.ptr(wptr)),
                    "pmx_ow_set_segments")
     model.cases = list(cases)
+    model.pieces = member_data
     model.segmented = segmented
     st = model.initial_state(c0.p0, c0.sw0)
     for c, (a, b) in zip(cases, ranges):
