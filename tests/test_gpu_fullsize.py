@@ -57,3 +57,59 @@ def test_full_size_step_conserves_mass_and_mixed_precision_matches_fp64():
     rel = np.abs(s32.pressure - s64.pressure).max() / np.abs(s64.pressure).max()
     assert rel <= 1e-6, rel
     assert np.abs(s32.sW - s64.sW).max() <= 1e-6
+
+
+def test_black_oil_full_size_step_conserves_mass_and_mixed_precision_matches_fp64():
+    """The black-oil workload at its full size (100x100x20, 200,000 cells,
+    gas cap, gas injector, four producers): over the second report step
+    (taken in one ministep) the change of water, oil and total gas (free and
+    dissolved) in place equals dt times the wells' surface rates, within the
+    MB and well tolerances; the FP32-preconditioner form reproduces the FP64
+    form's Newton counts, statuses and pressures (1e-6)."""
+    from oracle.bo import BOFluid
+    from pymrst_amd.nonlinear import NonLinearSolver
+    from pymrst_amd.ow import CPRSolver
+    from pymrst_amd.workloads import blackoil_field, build_bo_model
+    case = blackoil_field(steps=2)
+    runs = {}
+    for prec in (32, 64):
+        model, s0 = build_bo_model(case, CPRSolver(precision=prec))
+        states, rep = NonLinearSolver().simulate_schedule(model, s0, case.dts)
+        assert rep.converged
+        runs[prec] = (states, rep, model.pv_ref.copy())
+        del model
+        torch.cuda.empty_cache()
+    states, rep, pv = runs[32]
+    assert rep.steps[1].cuts == 0, "the second report step is one ministep"
+    f = BOFluid(case.deck)
+    dt = float(case.dts[1])
+
+    def in_place(st):
+        p = st.pressure
+        m = pv * f.pv_mult(p)
+        bo = f.b_o(p, st.rs, st.status == 3)
+        return np.array([np.sum(m * f.b_w(p) * st.s[:, 0]), np.sum(m * bo * st.s[:, 1]),
+                         np.sum(m * (f.b_g(p) * st.s[:, 2] + st.rs * bo * st.s[:, 1]))])
+
+    s1, s2 = states
+    dm = in_place(s2) - in_place(s1)
+    q = np.array([np.sum(s2.qWs), np.sum(s2.qOs), np.sum(s2.qGs)])
+    p = s2.pressure
+    bbar = np.array([np.mean(1.0 / f.b_w(p)), np.mean(1.0 / f.b_o(p, s2.rs, s2.status == 3)),
+                     np.mean(1.0 / f.b_g(p))])
+    # MB bounds the cell residuals by 1e-7 sum pv / B per phase; the well
+    # equations are held here to 1e-6 of the phase's largest well rate (the
+    # reference's own well test is looser: 1e-6 of the largest target, bhp
+    # included); 4x margins
+    q_ref = np.array([np.abs(s2.qWs).max(), np.abs(s2.qOs).max(), np.abs(s2.qGs).max()])
+    tol = 4e-7 * pv.sum() / bbar + 4 * dt * len(s2.qWs) * 1e-6 * q_ref
+    assert np.all(np.abs(dm - dt * q) <= tol), (dm, dt * q, tol)
+    # not vacuous: the oil and gas flows are far above the tolerance
+    gross = dt * np.array([np.abs(s2.qWs).sum(), np.abs(s2.qOs).sum(), np.abs(s2.qGs).sum()])
+    assert np.all(gross[1:] > 1e2 * tol[1:]), (gross, tol)
+    s64, rep64 = runs[64][0], runs[64][1]
+    assert [r.iterations for r in rep.steps] == [r.iterations for r in rep64.steps]
+    for a, b in zip(states, s64):
+        np.testing.assert_array_equal(a.status, b.status)
+        assert np.abs(a.pressure - b.pressure).max() <= 1e-6 * np.abs(b.pressure).max()
+        assert np.abs(a.s - b.s).max() <= 1e-6
