@@ -1,9 +1,10 @@
-"""The headline oil-water workload at its full size (60x220x85, 1,122,000
-cells), checked through properties that hold at any size: over a converged
-report step the change of water and of oil in place equals dt times the
-wells' surface rates (no-flow boundary; the MB tolerance bounds the
-mismatch), and the FP32-preconditioner form reproduces the FP64 form's
-Newton counts and pressures (1e-6 relative, the mixed-precision bar)."""
+"""The oil-water and black-oil workloads at their full sizes (60x220x85,
+1,122,000 cells; 100x100x20, 200,000 cells), checked through properties
+that hold at any size: over a converged report step the change of each
+phase's surface volume in place equals dt times the wells' surface rates
+(no-flow boundary; the MB and well tolerances bound the mismatch), and the
+FP32-preconditioner form reproduces the FP64 form's Newton counts and states
+(1e-6 relative in pressure, the mixed-precision bar)."""
 import numpy as np
 import pytest
 
