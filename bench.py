@@ -46,6 +46,9 @@ def spmv_bytes(nnzb: int, nc: int) -> float:
 # field case takes 1,480 FGMRES iterations instead of 1,796 at the earlier
 # 4 sweeps / omega 0.67, with the same Newton iterations
 OW_CPR = {"local_sweeps": 5, "jacobi_omega": 1.0}
+# the black-oil case (200k cells, 20-cell lines): three sweeps, both forms
+# (3,895 -> 4,065 FGMRES iterations but 2-4% faster than four; same Newton)
+BO_CPR = {"local_sweeps": 3}
 
 
 def run_form(case, precision, steps, warmup, rtol):
@@ -161,7 +164,7 @@ def blackoil_leg(steps: int = 20, cpu: bool = True):
     nc = case.num_cells
     out, p_last = {}, {}
     for prec in (64, 32):
-        solver = CPRSolver(rtol=1e-8, precision=prec)
+        solver = CPRSolver(rtol=1e-8, precision=prec, **BO_CPR)
         model, s0 = build_bo_model(case, solver)
         cuts = [0]
 
@@ -198,6 +201,7 @@ def blackoil_leg(steps: int = 20, cpu: bool = True):
     out["max_rel_pressure_diff_mixed_vs_fp64"] = float(np.max(np.abs(p_last[32] - p_last[64]) /
                                                               np.abs(p_last[64])))
     out["speedup_mixed_vs_fp64"] = round(out["fp32"]["value"] / out["fp64"]["value"], 3)
+    out["cpr"] = dict(BO_CPR)
     if cpu:
         out["cpu_baseline"] = bo_cpu_baseline()
     return out
