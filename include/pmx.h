@@ -267,6 +267,34 @@ int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const do
                      const double* wdp, const pmx_solver_options* o, double* p, double* flux, double* face_pressure,
                      double* well_p, double* well_flux, pmx_solve_info* info);
 
+/* ---- a caller's linear system -----------------------------------------
+ * Replaces the reference's linear-solver seam: CPRSolver.solve(jac, rhs,
+ * model, state, dt) (pymrst/modules/ad_core/solvers/linear_solvers.py:426-545)
+ * as called from NonLinearSolver._solve_linear (simulators/nonlinear_solver.py
+ * :383-423) and TwoPhaseOilWaterModelAD._solve_increment
+ * (two_phase_oil_water_model.py:597-603), for a Jacobian assembled by the
+ * caller.  Cell unknowns cell-major in B x B blocks (B = 2 oil-water, 3
+ * black-oil), then M = B + 1 unknowns per well (B surface rates, bhp).
+ * Host: rowptr[nc+1], col[nnzb] (ascending per row, every diagonal block
+ * present), perf_ptr[nw+1], perf_cells[nperf]. */
+typedef struct pmx_sys pmx_sys;
+int pmx_sys_create(int32_t device, void* hip_stream, int32_t B, int32_t nc, const int32_t* rowptr,
+                   const int32_t* col, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells,
+                   pmx_sys** out);
+int pmx_sys_destroy(pmx_sys* h);
+/* Device: val[nnzb*B*B] (row-major blocks), Bp[nperf*B*M] (a perforated
+ * cell's rows against its well's unknowns), Cp[nperf*M*B] (the well's rows
+ * against the cell), Dp[nw*M*M].  The first call also builds the pressure
+ * aggregation and the lines from |a_pp| of these values. */
+int pmx_sys_set_values(pmx_sys* h, const double* val, const double* Bp, const double* Cp, const double* Dp);
+/* y = J x (device vectors of nc*B + nw*M). */
+int pmx_sys_matvec(pmx_sys* h, const double* x, double* y);
+/* J x = rhs by CPR-FGMRES; w[B] (host) are the quasi-IMPES equation weights
+ * of CPRSolver._weighting (linear_solvers.py:155-190).  Returns 0 converged,
+ * 1 not converged, <0 error. */
+int pmx_sys_solve(pmx_sys* h, const pmx_solver_options* o, const double* w, const double* rhs, double* x,
+                  pmx_solve_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,11 @@ SIGNATURES = {
     "pmx_ow_profile": (ctypes.c_int, [_P, _I]),
     "pmx_ow_profile_read": (ctypes.c_int, [_P, _P]),
     "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
+    "pmx_sys_create": (ctypes.c_int, [_I, _P, _I, _I, _P, _P, _I, _P, _P, ctypes.POINTER(_P)]),
+    "pmx_sys_destroy": (ctypes.c_int, [_P]),
+    "pmx_sys_set_values": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "pmx_sys_matvec": (ctypes.c_int, [_P, _P, _P]),
+    "pmx_sys_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P, _P, ctypes.POINTER(SolveInfo)]),
     "pmx_ow_set_trans_fractions": (ctypes.c_int, [_P, _P]),
     "pmx_ow_set_segments": (ctypes.c_int, [_P, _I, _P, _P]),
     "pmx_ow_mask_residual": (ctypes.c_int, [_P, _P]),

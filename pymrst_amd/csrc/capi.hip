@@ -2,6 +2,8 @@
 // each entry point converts them into a negative status and a message.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cmath>
 #include <string>
 
 #include "../../include/pmx.h"
@@ -26,6 +28,21 @@ struct pmx_bo {
   int device = 0;
   std::unique_ptr<BOModelDev> m;
   pmx_solver_options last_opts{};
+};
+
+// A block system given by its caller: cell blocks in BCSR (any pattern that
+// holds every diagonal block), per-perforation well blocks.
+struct pmx_sys {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  BlockSystem S;
+  CprSolver* cpr = nullptr;
+  DArr<double> w;                 // pressure weights (B)
+  std::vector<int> h_rowptr, h_col;
+  bool have_values = false;
+  ~pmx_sys() {
+    if (cpr) cpr_destroy(cpr);
+  }
 };
 
 static thread_local std::string g_err;
@@ -653,6 +670,149 @@ int pmx_bo_profile_read(pmx_bo* h, double* out4) {
     bind(h);
     cpr_profile_read(h->m->cpr, out4);
   })
+}
+
+
+int pmx_sys_create(int32_t device, void* hip_stream, int32_t B, int32_t nc, const int32_t* rowptr,
+                   const int32_t* col, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells,
+                   pmx_sys** out) {
+  try {
+    PMX_CHECK(out != nullptr && rowptr != nullptr && col != nullptr, "null argument");
+    PMX_CHECK(B == 2 || B == 3, "block size must be 2 (oil-water) or 3 (black-oil)");
+    PMX_CHECK(nc > 0 && nw >= 0, "empty system");
+    PMX_CHECK(nw == 0 || (perf_ptr != nullptr && perf_cells != nullptr), "wells need their perforations");
+    auto h = std::make_unique<pmx_sys>();
+    h->device = device;
+    PMX_HIP(hipSetDevice(device));
+    h->stream = (hipStream_t)hip_stream;
+    h->S.B = B;
+    h->S.M = B + 1;
+    h->S.X = B;
+    HostPattern pat;
+    pat.n = nc;
+    pat.rowptr.assign(rowptr, rowptr + nc + 1);
+    PMX_CHECK(pat.rowptr[0] == 0, "rowptr must start at 0");
+    const int nnzb = pat.rowptr[nc];
+    pat.col.assign(col, col + nnzb);
+    pat.diag.assign(nc, -1);
+    for (int r = 0; r < nc; ++r) {
+      PMX_CHECK(pat.rowptr[r + 1] >= pat.rowptr[r], "rowptr must not decrease");
+      for (int k = pat.rowptr[r]; k < pat.rowptr[r + 1]; ++k) {
+        PMX_CHECK(pat.col[k] >= 0 && pat.col[k] < nc, "block column out of range");
+        PMX_CHECK(k == pat.rowptr[r] || pat.col[k] > pat.col[k - 1], "block columns must ascend within a row");
+        if (pat.col[k] == r) pat.diag[r] = k;
+      }
+      PMX_CHECK(pat.diag[r] >= 0, "every block row needs its diagonal block");
+    }
+    h->S.set_pattern(pat, h->stream);
+    std::vector<int> pp(nw + 1, 0), pc;
+    if (nw > 0) {
+      pp.assign(perf_ptr, perf_ptr + nw + 1);
+      PMX_CHECK(pp[0] == 0, "perf_ptr must start at 0");
+      for (int w = 0; w < nw; ++w) PMX_CHECK(pp[w + 1] > pp[w], "every well needs a perforation");
+      pc.assign(perf_cells, perf_cells + pp[nw]);
+    }
+    h->S.set_wells(nw, pp, pc, h->stream);
+    h->S.has_mix = false;
+    h->h_rowptr = pat.rowptr;
+    h->h_col = pat.col;
+    h->cpr = cpr_create();
+    h->w.alloc(B);
+    PMX_HIP(hipStreamSynchronize(h->stream));
+    *out = h.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pmx_sys_destroy(pmx_sys* h) {
+  try {
+    if (h) {
+      PMX_HIP(hipSetDevice(h->device));
+      delete h;
+    }
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pmx_sys_set_values(pmx_sys* h, const double* val, const double* Bp, const double* Cp, const double* Dp) {
+  try {
+    PMX_CHECK(h != nullptr && val != nullptr, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    BlockSystem& S = h->S;
+    const size_t B = S.B, M = S.M;
+    PMX_HIP(hipMemcpyAsync(S.val.p, val, sizeof(double) * S.nnzb * B * B, hipMemcpyDeviceToDevice, h->stream));
+    if (S.nw > 0) {
+      PMX_CHECK(Bp != nullptr && Cp != nullptr && Dp != nullptr, "a system with wells needs Bp, Cp and Dp");
+      PMX_HIP(hipMemcpyAsync(S.Bp.p, Bp, sizeof(double) * S.nperf * B * M, hipMemcpyDeviceToDevice, h->stream));
+      PMX_HIP(hipMemcpyAsync(S.Cp.p, Cp, sizeof(double) * S.nperf * M * B, hipMemcpyDeviceToDevice, h->stream));
+      PMX_HIP(hipMemcpyAsync(S.Dp.p, Dp, sizeof(double) * S.nw * M * M, hipMemcpyDeviceToDevice, h->stream));
+    }
+    if (!h->have_values) {
+      // static aggregation from the first values: |a_pp| of each block
+      std::vector<double> pp(S.nnzb);
+      std::vector<double> blk((size_t)S.nnzb * B * B);
+      PMX_HIP(hipMemcpyAsync(blk.data(), S.val.p, sizeof(double) * blk.size(), hipMemcpyDeviceToHost, h->stream));
+      PMX_HIP(hipStreamSynchronize(h->stream));
+      for (int k = 0; k < S.nnzb; ++k) pp[k] = std::fabs(blk[(size_t)k * B * B]);
+      LinearSolverOptions o;
+      cpr_build_hierarchy(h->cpr, S, pp, o, h->stream);
+      h->have_values = true;
+    }
+    PMX_HIP(hipStreamSynchronize(h->stream));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pmx_sys_matvec(pmx_sys* h, const double* x, double* y) {
+  try {
+    PMX_CHECK(h != nullptr && x != nullptr && y != nullptr, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    DArr<double> scratch;
+    scratch.alloc(std::max<size_t>((size_t)h->S.nw * h->S.M, 1));
+    system_matvec(h->S, x, y, scratch.p, h->stream);
+    PMX_HIP(hipStreamSynchronize(h->stream));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return 0;
+}
+
+int pmx_sys_solve(pmx_sys* h, const pmx_solver_options* opts, const double* w, const double* rhs, double* x,
+                  pmx_solve_info* info) {
+  int status = 0;
+  try {
+    PMX_CHECK(h != nullptr && w != nullptr && rhs != nullptr && x != nullptr, "null argument");
+    PMX_CHECK(h->have_values, "set the values before solving");
+    PMX_HIP(hipSetDevice(h->device));
+    LinearSolverOptions o = to_opts(opts);
+    PMX_HIP(hipMemcpyAsync(h->w.p, w, sizeof(double) * h->S.B, hipMemcpyHostToDevice, h->stream));
+    auto t0 = std::chrono::steady_clock::now();
+    cpr_setup(h->cpr, h->S, h->w.p, o, h->stream);
+    PMX_HIP(hipStreamSynchronize(h->stream));
+    const double setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    SolveInfo si;
+    status = cpr_solve(h->cpr, h->S, rhs, x, o, si, h->stream);
+    si.setup_ms = setup_ms;
+    fill_info(info, si);
+    if (status < 0) {
+      g_err = "linear solve produced a non-finite residual";
+      return -2;
+    }
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+  return status;
 }
 
 }  // extern "C"

@@ -118,6 +118,15 @@ class CPRSolver:
     last_iterations: int = 0
     history: list = field(default_factory=list)
 
+    def solve(self, jac, rhs, model=None, state=None, dt=None):
+        """The reference's seam (linear_solvers.py:426-545): solve the
+        caller's Jacobian (SciPy sparse, the reference's variable-blocked
+        order) by CPR-FGMRES on the device; returns the increment as a NumPy
+        array.  Non-convergence raises RuntimeError; a model without
+        .G, .n_phases and .fluid.{disgas, vapoil, b} raises ValueError."""
+        from .linear import solve_reference_system
+        return solve_reference_system(self, jac, rhs, model, state)
+
     def options(self) -> _lib.SolverOptions:
         o = _lib.solver_defaults()
         o.rtol, o.maxiter, o.restart, o.precision = self.rtol, self.maxiter, self.restart, self.precision
