@@ -44,6 +44,12 @@ void tpfa_pattern(int n, int nconn, const int* N, HostPattern& pat, std::vector<
       return x.first < y.first;
     });
     for (int k = pat.rowptr[i]; k < pat.rowptr[i + 1]; ++k) {
+      // one connection per block entry: the assembly reads one T and one
+      // gravity drop per entry.  Two connections between the same cells
+      // carry the same gravity drop and upwind direction, so their sum is
+      // one connection with the summed transmissibility.
+      PMX_CHECK(k == pat.rowptr[i] || ent[k].first != ent[k - 1].first,
+                "connection list holds a cell pair twice (merge them into one connection with the summed T)");
       pat.col[k] = ent[k].first;
       entry_conn[k] = ent[k].second;
       if (ent[k].first == i) pat.diag[i] = k;

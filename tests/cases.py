@@ -63,7 +63,32 @@ def small_gravity_inputs(seed=0, nx=6, ny=5, nz=4):
 
 
 # ---------------------------------------------------------------------------
-def build_oracle(inp, linear_solver=None):
+def faulted_connections(g, N, T, frac=0.5):
+    """A Cartesian grid's connection list with a normal fault at x = nx/2,
+    as a corner-point grid with displaced columns presents it: for j < ny/2
+    the x-connections across the fault are dropped and cell (i0-1, j, k)
+    connects to (i0, j, k+1) instead, through `frac` of the dropped
+    transmissibility (non-neighbouring pairs, appended after the regular
+    connections as MRST appends NNCs); one long-range pair, listed as
+    (last cell, first cell), models a weak leak between the corners."""
+    nx, ny, nz = g.dims
+    i0 = nx // 2
+    N = np.asarray(N)
+    T = np.asarray(T, float)
+    a = N[:, 0]
+    i, j, k = a % nx, (a // nx) % ny, a // (nx * ny)
+    cut = (N[:, 1] - a == 1) & (i == i0 - 1) & (j < ny // 2)
+    keep = ~cut
+    nnc = cut & (k < nz - 1)
+    extra = np.column_stack([a[nnc], a[nnc] + 1 + nx * ny])
+    t_extra = frac * T[nnc]
+    leak = np.array([[g.num_cells - 1, 0]])
+    N2 = np.vstack([N[keep], extra, leak]).astype(N.dtype)
+    T2 = np.concatenate([T[keep], t_extra, [1e-3 * np.median(T)]])
+    return N2, T2
+
+
+def build_oracle(inp, linear_solver=None, connections=None):
     g = og.cart_grid(inp["dims"], inp["phys"])
     rock = og.make_rock(g, inp["perm"], inp["poro"])
     grav = np.array([0, 0, inp["gravity"]]) if inp["gravity"] else None
@@ -74,11 +99,13 @@ def build_oracle(inp, linear_solver=None):
                  n=np.asarray(inp["n"], float), c=np.asarray(inp["c"], float), p_ref=inp["p_ref"],
                  c_r=inp["c_r"])
     N, T, _, _ = og.internal_connections(g, og.compute_trans(g, rock))
+    if connections is not None:
+        N, T = connections(g, N, T)
     m = OWModel(g, rock, fl, wells, N, T, gravity=inp["gravity"], linear_solver=linear_solver)
     return m, m.initial_state(inp["p0"], inp["sw0"])
 
 
-def build_device(inp, linear_solver=None):
+def build_device(inp, linear_solver=None, connections=None):
     import pymrst_amd as pa
     from pymrst_amd import ow
     g = pa.cart_grid(inp["dims"], inp["phys"])
@@ -88,7 +115,13 @@ def build_device(inp, linear_solver=None):
     for cells, kind, val, compi, sign in inp["wells"]:
         wells = pa.add_well(wells, g, rock, cells, kind, val, compi=np.array(compi), sign=sign, gravity=grav)
     fl = ow.init_simple_adi_fluid(inp["mu"], inp["rho"], inp["n"], inp["c"], inp["p_ref"], c_r=inp["c_r"])
-    m = ow.TwoPhaseOilWaterModelAD(g, rock, fl, wells, gravity=inp["gravity"], linear_solver=linear_solver)
+    N = T = None
+    if connections is not None:
+        go = og.cart_grid(inp["dims"], inp["phys"])
+        N, T = og.internal_connections(go, og.compute_trans(go, og.make_rock(go, inp["perm"], inp["poro"])))[:2]
+        N, T = connections(go, N, T)
+    m = ow.TwoPhaseOilWaterModelAD(g, rock, fl, wells, gravity=inp["gravity"], linear_solver=linear_solver,
+                                   N=N, T=T)
     return m, m.initial_state(inp["p0"], inp["sw0"])
 
 
