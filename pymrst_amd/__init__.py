@@ -30,4 +30,7 @@ def __getattr__(name):
                 "init_deck_blackoil_fluid"):
         from . import bo
         return getattr(bo, name)
+    if name in ("ResultHandler", "simulate_with_output"):
+        from . import results
+        return getattr(results, name)
     raise AttributeError(name)

@@ -245,6 +245,26 @@ class TwoPhaseOilWaterModelAD:
         self._cdp_dev.zero_()
         self.stats = {"newton": 0, "linear": 0, "assemble_s": 0.0, "solve_s": 0.0}
 
+    def run_memory(self) -> dict:
+        """What a run carries from one step to the next besides the state:
+        the previous step's per-connection well cdp, which seeds the next
+        step's connection rates (the reference keeps it on the model,
+        two_phase_oil_water_model.py:254-283).  A restart that restores it
+        continues the run exactly."""
+        return {} if self._last_cdp is None else {"well_cdp": [np.array(c, float) for c in self._last_cdp]}
+
+    def restore_run_memory(self, memory: Optional[dict]):
+        cdp = (memory or {}).get("well_cdp")
+        if cdp is None:
+            self._last_cdp = None
+            self._cdp_dev.zero_()
+            return
+        if len(cdp) != self.nw or any(np.size(c) != w.cells.size for c, w in zip(cdp, self.wells)):
+            raise ValueError("stored well cdp does not match this model's wells")
+        self._last_cdp = [np.asarray(c, float) for c in cdp]
+        flat = np.concatenate(self._last_cdp)
+        self._cdp_dev[:flat.size].copy_(torch.as_tensor(flat, dtype=torch.float64))
+
     def profile(self, enable: bool):
         _lib.check(self.lib.pmx_ow_profile(self.h, int(enable)), "pmx_ow_profile")
 

@@ -1,0 +1,187 @@
+"""The result store and restart path (CPU): the reference's own
+ResultHandler tests (pymrst/modules/ad_core/utils/result_handler.py:335-380:
+memory and disk round trips, restart from an empty store, reset of all or
+some ids), the pickle-free wire format (states, dicts, member lists, dtypes
+kept bit for bit; files load with allow_pickle=False), and a schedule
+interrupted after two report steps and resumed from disk by a fresh model
+and handler, which reproduces the uninterrupted run bit for bit once the
+model's run memory (the previous well cdp) is restored from the report store
+(on the oracle, the driver injected)."""
+import dataclasses
+import zipfile
+
+import numpy as np
+import pytest
+
+import oracle.ow as oow
+from cases import build_oracle, small_gravity_inputs
+
+
+@pytest.fixture(autouse=True)
+def _oracle_states():
+    from pymrst_amd import results
+    results.register(oow.OWState, "oracle.OWState")
+    yield
+    results._REGISTRY.pop("oracle.OWState", None)
+
+
+def test_memory_round_trip():
+    from pymrst_amd.results import ResultHandler
+    h = ResultHandler()
+    h[1] = {"time": 0.0, "pressure": 1.0}
+    h[2] = {"time": 10.0, "pressure": 2.0}
+    assert h[2]["pressure"] == 2.0
+    assert h.get_valid_ids() == [1, 2]
+
+
+def test_disk_round_trip_and_gaps(tmp_path):
+    from pymrst_amd.results import ResultHandler
+    h = ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False)
+    h[1] = {"time": 0.0, "x": np.array([1, 2, 3], np.int32)}
+    h[3] = {"time": 2.0, "x": np.array([4], np.int32)}
+    fresh = ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False)
+    x = fresh[1]["x"]
+    assert x.dtype == np.int32 and x.tolist() == [1, 2, 3]
+    assert fresh.get_valid_ids() == [1, 3] and fresh.numel_data() == 3 and len(fresh) == 2
+    assert fresh.size_data() == (3, 1) and 2 not in fresh and 3 in fresh
+    with pytest.raises(KeyError):
+        fresh[2]
+
+
+def test_restart_from_an_empty_store():
+    from pymrst_amd.results import ResultHandler
+    state, step = ResultHandler().get_restart()
+    assert state is None and step == 1
+
+
+def test_reset_data_all_or_some(tmp_path):
+    from pymrst_amd.results import ResultHandler
+    h = ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False)
+    for i in range(1, 5):
+        h[i] = {"time": float(i)}
+    h.reset_data([2, 3])
+    assert h.get_valid_ids() == [1, 4]
+    h.reset_data(np.array([True, False, False, False]))
+    assert h.get_valid_ids() == [4]
+    h.reset_data()
+    assert h.get_valid_ids() == []
+    m = ResultHandler()
+    m[1], m[2] = {"time": 0.0}, {"time": 1.0}
+    m.reset_data([1])
+    assert m.get_valid_ids() == [2]
+
+
+def test_search_by_time_and_predicate():
+    from pymrst_amd.results import ResultHandler
+    h = ResultHandler()
+    for i, t in enumerate([0.0, 5.0, 10.0], start=1):
+        h[i] = {"time": t, "k": i}
+    assert h.get_by_time(5.0)[1] == 2
+    assert h.get_by_time(7.0) == (None, None)
+    s, i, d = h.get_closest_by_time(7.0)
+    assert i == 2 and d == 2.0
+    assert h.get_first_by_function(lambda s: s["k"] > 1, reverse=True)[1] == 3
+
+
+def test_nothing_to_keep_and_paths_outside_are_errors(tmp_path):
+    from pymrst_amd.results import ResultHandler
+    with pytest.raises(ValueError, match="discarded"):
+        ResultHandler(store_in_memory=False)[1] = {"time": 0.0}
+    (tmp_path / "results").mkdir()
+    outside = ResultHandler(tmp_path, data_prefix="../state", write_to_disk=True, store_in_memory=False)
+    with pytest.raises(ValueError, match="outside"):
+        outside.read_from_file(1)
+    with pytest.raises(ValueError, match=".npz"):
+        ResultHandler(extension=".pkl")
+
+
+def test_wire_format_is_plain_arrays_and_exact(tmp_path):
+    from pymrst_amd.ow import OWState
+    from pymrst_amd.bo import BOState
+    from pymrst_amd.results import ResultHandler, decode, encode
+    rng = np.random.default_rng(0)
+    ow = OWState(*(rng.random(n) for n in (7, 7, 2, 2, 2)))
+    bo = BOState(rng.random(5), rng.random((5, 3)), rng.random(5), np.array([1, 3, 3, 1, 1], np.int32),
+                 rng.random(2), rng.random(2), rng.random(2), rng.random(2))
+    members = [ow, OWState(*(rng.random(n) for n in (4, 4, 1, 1, 1)))]
+    step = {"time": 3.5, "members": members, "bo": bo, "tag": "ensemble", "none": None, "pair": (1, 2.5)}
+    h = ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False)
+    h[1] = step
+    path = h.get_data_path(1)
+    with zipfile.ZipFile(path) as z:
+        assert all(n.endswith(".npy") for n in z.namelist())
+    with np.load(path, allow_pickle=False) as z:        # no object arrays anywhere
+        assert all(z[k].dtype != object for k in z.files)
+    back = ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False)[1]
+    assert back["time"] == 3.5 and back["tag"] == "ensemble" and back["none"] is None
+    assert back["pair"] == (1, 2.5)
+    for a, b in zip(back["members"], members):
+        assert isinstance(a, OWState)
+        for f in dataclasses.fields(b):
+            np.testing.assert_array_equal(getattr(a, f.name), getattr(b, f.name))
+    assert back["bo"].status.dtype == np.int32
+    np.testing.assert_array_equal(back["bo"].s, bo.s)
+    assert decode(encode(ow)).pressure.tobytes() == ow.pressure.tobytes()
+    with pytest.raises(TypeError):
+        encode({"f": object()})
+
+
+def test_unregistered_kinds_are_not_rebuilt():
+    from pymrst_amd.results import decode, encode
+
+    @dataclasses.dataclass
+    class Stranger:
+        x: np.ndarray
+
+    arrays = encode(Stranger(np.zeros(2)))
+    with pytest.raises(ValueError, match="not registered"):
+        decode(arrays)
+
+
+def _remembering(m):
+    """The oracle model with the run-memory hooks of the device model (its
+    previous step's well cdp, which it keeps as the reference does)."""
+    m.run_memory = lambda: {} if m._last_cdp is None else {"well_cdp": [np.array(c) for c in m._last_cdp]}
+
+    def restore(mem):
+        c = (mem or {}).get("well_cdp")
+        m._last_cdp = None if c is None else [np.asarray(x, float) for x in c]
+    m.restore_run_memory = restore
+    return m
+
+
+def test_interrupted_schedule_resumes_exactly(tmp_path):
+    from pymrst_amd.results import ResultHandler, simulate_with_output
+    inp = small_gravity_inputs()
+    dts = np.concatenate([inp["dts"], [20 * 86400.0]])
+    mo, s0 = build_oracle(inp)
+    full, full_its = oow.simulate_schedule(s0, mo, dts)
+
+    def handlers(root):
+        return (ResultHandler(root, write_to_disk=True, store_in_memory=False),
+                ResultHandler(root, data_prefix="report", write_to_disk=True, store_in_memory=False))
+
+    st, rep = handlers(tmp_path)
+    part, its = simulate_with_output(s0, _remembering(build_oracle(inp)[0]), dts[:2], st, rep,
+                                     simulate=oow.simulate_schedule)
+    assert st.get_valid_ids() == [1, 2] and its.tolist() == full_its[:2].tolist()
+    st, rep = handlers(tmp_path)               # a new process: only the files remain
+    states, its = simulate_with_output(s0, _remembering(build_oracle(inp)[0]), dts, st, rep,
+                                       simulate=oow.simulate_schedule)
+    assert its[:2].tolist() == [0, 0] and its[2:].tolist() == full_its[2:].tolist()
+    assert st.get_valid_ids() == [1, 2, 3, 4]
+    for a, b in zip(states, full):
+        assert a.pressure.tobytes() == b.pressure.tobytes() and a.sW.tobytes() == b.sW.tobytes()
+        assert a.bhp.tobytes() == b.bhp.tobytes()
+    assert rep.get_by_time(float(dts[:3].sum()))[1] == 3
+    assert rep[4]["iterations"] == int(full_its[3])
+    # the stored memory matters: resumed from the states alone (a fresh
+    # model's zero cdp), the gravity wells' first rates differ
+    alone = ResultHandler(tmp_path / "alone", write_to_disk=True, store_in_memory=False)
+    alone[1], alone[2] = full[0], full[1]
+    loose, _ = simulate_with_output(s0, build_oracle(inp)[0], dts, alone, simulate=oow.simulate_schedule)
+    assert loose[2].pressure.tobytes() != full[2].pressure.tobytes()
+    st2, _ = handlers(tmp_path)
+    st2[7] = full[0]
+    with pytest.raises(ValueError, match="contiguous"):
+        simulate_with_output(s0, mo, dts, st2, simulate=oow.simulate_schedule)
