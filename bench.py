@@ -31,6 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # calibrated against a 512 MiB copy (fetch factor 2.0, write factor 1.0):
 # 404.3 MB read + 17.95 MB written.
 MEASURED_SPMV_TRAFFIC = {(60, 220, 85): 422.2e6}
+# k_line_gs<2,float,16> per launch (one colour), FETCH_SIZE x 2 + WRITE_SIZE: 47.49 + 31.14 MB
+MEASURED_LINE_GS_TRAFFIC = {(60, 220, 85): 78.64e6}
 
 
 def spmv_bytes(nnzb: int, nc: int) -> float:
@@ -462,6 +464,24 @@ def main():
                 "kernel": "k_bcsr_spmv<2,8> (FP64 2x2 BCSR SpMV of the Jacobian)",
                 "bytes_per_launch": nbytes, "avg_launch_us": round(avg_s * 1e6, 2),
                 "launches": prof["spmv_count"]}
+    # the kernel with the most GPU time is the line solve of the CPR local
+    # stage; it runs inside the replayed preconditioner graph, so it is timed
+    # launch by launch in eager applications on the last Newton system
+    roof_spmv, roof = roof, None
+    if head["model"].linear_solver.local_stage == "line":
+        lg = head["model"].time_line_solve(20)
+        if lg["launches"] > 0:
+            avg_s = lg["ms"] / lg["launches"] / 1e3
+            per_launch = lg["bytes"] / lg["launches"]
+            achieved = per_launch / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_LINE_GS_TRAFFIC.get(dims),
+                    "kernel": "k_line_gs<2,float,16> (block-tridiagonal line solves of one colour, the CPR "
+                              "local stage; the kernel with the most GPU time)",
+                    "bytes_per_launch": round(per_launch), "avg_launch_us": round(avg_s * 1e6, 2),
+                    "launches": lg["launches"]}
+    if roof is None:
+        roof, roof_spmv = roof_spmv, None
     cpu = None
     if not args.no_cpu_baseline:
         sample = field_waterflood(dims=(dims[0], max(2, dims[1] // 10), dims[2]))
@@ -474,7 +494,7 @@ def main():
             "config": {"workload": f"oil-water field waterflood {dims[0]}x{dims[1]}x{dims[2]} "
                                    f"({nc} cells), five-spot, gravity, {steps} x 30 d, CPR-FGMRES rtol {args.rtol}",
                        "cells": nc, "report_steps": steps, "cpr": dict(OW_CPR)},
-            "roofline": roof, "cpu_baseline": cpu}
+            "roofline": roof, "roofline_spmv": roof_spmv, "cpu_baseline": cpu}
     line.update(extra)
     line["stream_copy"] = stream_copy()
     if not args.no_blackoil:

@@ -130,6 +130,12 @@ int pmx_ow_set_lines(pmx_ow* h, int32_t nlines, const int32_t* line_ptr, const i
  * total ms, preconditioner applications, their total ms}. */
 int pmx_ow_profile(pmx_ow* h, int32_t enable);
 int pmx_ow_profile_read(pmx_ow* h, double* out4);
+/* The line Gauss-Seidel solve kernel timed launch by launch with HIP events
+ * while profiling is on, in preconditioner applications that run without
+ * graph replay (pmx_ow_precondition with use_graphs = 0); out3 (host) =
+ * {launches timed, their total ms, their algorithmic bytes}.  No counterpart
+ * in the reference (measurement only). */
+int pmx_ow_profile_line_gs(pmx_ow* h, double* out3);
 
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);

@@ -74,6 +74,7 @@ SIGNATURES = {
     "pmx_ow_set_lines": (ctypes.c_int, [_P, _I, _P, _P]),
     "pmx_ow_profile": (ctypes.c_int, [_P, _I]),
     "pmx_ow_profile_read": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_profile_line_gs": (ctypes.c_int, [_P, _P]),
     "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
     "pmx_sys_create": (ctypes.c_int, [_I, _P, _I, _I, _P, _P, _I, _P, _P, ctypes.POINTER(_P)]),
     "pmx_sys_destroy": (ctypes.c_int, [_P]),

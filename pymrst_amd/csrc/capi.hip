@@ -315,6 +315,14 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4) {
   })
 }
 
+int pmx_ow_profile_line_gs(pmx_ow* h, double* out3) {
+  PMX_TRY({
+    PMX_CHECK(h && out3, "null argument");
+    bind(h);
+    cpr_profile_line_read(h->m->cpr, out3);
+  })
+}
+
 int pmx_incomp_create(int32_t device, void* hip_stream, pmx_incomp** out) {
   PMX_TRY({
     PMX_CHECK(out != nullptr, "null argument");

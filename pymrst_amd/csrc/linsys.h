@@ -48,6 +48,10 @@ struct EventTimer {
   size_t used = 0;
   void start(hipStream_t s);
   void stop(hipStream_t s);
+  // the next (start, stop) pair for a launch that stamps them itself
+  // (hipExtLaunchKernelGGL: the dispatch's own begin and end times, no
+  // separate event packets); false when the timer is off
+  bool pair(hipEvent_t* a, hipEvent_t* b);
   double total_ms(int* count);   // synchronises on the last event
   void reset() { used = 0; }
   ~EventTimer();
@@ -120,6 +124,8 @@ void cpr_apply(CprSolver*, const BlockSystem& S, const double* r, double* z, con
 void cpr_profile(CprSolver*, bool on);
 // out[0..3] = {spmv launches, spmv ms, preconditioner applications, preconditioner ms}
 void cpr_profile_read(CprSolver*, double* out);
+// line-solve launches timed in eager applications: count, ms, algorithmic bytes
+void cpr_profile_line_read(CprSolver*, double* out3);
 
 // Helpers used by several sources.
 int grid_for(size_t threads, int block = 256);

@@ -12,6 +12,7 @@
 // pyamg smoothed aggregation is not available to pin against (parity is
 // stated on the solution of J x = b, as the reference's own tests do,
 // linear_solvers.py:639-649).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -255,6 +256,18 @@ void EventTimer::stop(hipStream_t s) {
   PMX_HIP(hipEventRecord(ev[used + 1], s));
   used += 2;
 }
+bool EventTimer::pair(hipEvent_t* a, hipEvent_t* b) {
+  if (!on) return false;
+  while (ev.size() < used + 2) {
+    hipEvent_t e;
+    PMX_HIP(hipEventCreate(&e));
+    ev.push_back(e);
+  }
+  *a = ev[used];
+  *b = ev[used + 1];
+  used += 2;
+  return true;
+}
 double EventTimer::total_ms(int* count) {
   double t = 0.0;
   if (used) PMX_HIP(hipEventSynchronize(ev[used - 1]));
@@ -272,10 +285,13 @@ static void matvec_t(const BlockSystem& S, const double* x, double* y, double* h
   const size_t ncB = (size_t)S.nc * B;
   constexpr int LPR = 8;
   if (S.nc > 0) {
-    if (t) t->start(s);
-    hipLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                       S.rowptr.p, S.col.p, S.val.p, x, y);
-    if (t) t->stop(s);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (t && t->pair(&e0, &e1))
+      hipExtLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, e0, e1, 0,
+                            S.nc, S.rowptr.p, S.col.p, S.val.p, x, y);
+    else
+      hipLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                         S.rowptr.p, S.col.p, S.val.p, x, y);
     launch_check();
   }
   if (S.nw > 0) {
@@ -1474,6 +1490,12 @@ class CprSolver {
   const double* pcdev = nullptr;      // prolongation weights of the current setup (column spread)
   DArr<double> pc_unit;               // e_0: the pressure column
   EventTimer spmv_t, prec_t;
+  // line-solve launches timed in eager applications (never inside a
+  // capture), with the algorithmic bytes of the launches timed
+  EventTimer lgs_t;
+  double lgs_bytes = 0.0;
+  bool capturing = false;
+  std::vector<long long> color_cells;                // cells on the lines of each colour
   // line local stage: position-major line tables and block-tridiagonal factors
   int nlines = 0, maxlen = 0, longest = 0, ncolor = 0;
   std::vector<int> color_ptr, color_len;             // line range and longest line of each colour
@@ -1505,6 +1527,16 @@ void cpr_profile(CprSolver* C, bool on) {
   C->prec_t.on = on;
   C->spmv_t.reset();
   C->prec_t.reset();
+  C->lgs_t.on = on;
+  C->lgs_t.reset();
+  C->lgs_bytes = 0.0;
+}
+
+void cpr_profile_line_read(CprSolver* C, double* out) {
+  int n = 0;
+  out[1] = C->lgs_t.total_ms(&n);
+  out[0] = n;
+  out[2] = C->lgs_bytes;
 }
 
 void cpr_profile_read(CprSolver* C, double* out) {
@@ -1608,6 +1640,9 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   C->ncolor = HL.ncolor;
   C->color_ptr = HL.color_ptr;
   C->color_len.assign(HL.ncolor, 0);
+  C->color_cells.assign(HL.ncolor, 0);
+  for (int c = 0; c < HL.ncolor; ++c)
+    for (int l = HL.color_ptr[c]; l < HL.color_ptr[c + 1]; ++l) C->color_cells[c] += HL.len[l];
   for (int c = 0; c < HL.ncolor; ++c)
     if (HL.color_ptr[c + 1] > HL.color_ptr[c]) C->color_len[c] = HL.len[HL.color_ptr[c]];
   C->ln_len.upload(HL.len, s);
@@ -1966,10 +2001,26 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
                            nlc, Lc, c, sw == 0 ? 1 : 0, C->noff, npos, C->ln_len.p, C->off_nbr.p, C->off_col.p,
                            V::aoff(*C).p, V::yc(*C).p, V::r1l(*C).p, V::r2(*C).p);
         launch_check();
-        hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
-                           C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
-                           V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool timed = !C->capturing && C->lgs_t.pair(&e0, &e1);
+        if (timed)
+          hipExtLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, e0, e1, 0,
+                                C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
+                                V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p,
+                                C->pcdev);
+        else
+          hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
+                             C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
+                             V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
         launch_check();
+        if (timed) {
+          // per line cell: forward D'^-1, L, r read and y written; backward
+          // D'^-1, U, y, the pressure xp and the cell index read, z (FP64)
+          // written, and y_cell unless this is the last launch
+          const double sv = sizeof(T);
+          const double per_cell = 4.0 * B * B * sv + 3.0 * B * sv + sv + 4.0 + 8.0 * B + (last ? 0.0 : B * sv);
+          C->lgs_bytes += per_cell * (double)C->color_cells[c];
+        }
       }
     }
     return;
@@ -2023,9 +2074,12 @@ static void capture_apply(CprSolver* C, const BlockSystem& S, const LinearSolver
   }
   PMX_HIP(hipStreamBeginCapture(C->cap_s, hipStreamCaptureModeThreadLocal));
   hipGraph_t g = nullptr;
+  C->capturing = true;
   try {
     apply_sequence(C, S, o, C->cap_s);
+    C->capturing = false;
   } catch (...) {
+    C->capturing = false;
     (void)hipStreamEndCapture(C->cap_s, &g);
     if (g) (void)hipGraphDestroy(g);
     throw;

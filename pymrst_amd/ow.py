@@ -274,6 +274,30 @@ class TwoPhaseOilWaterModelAD:
         return {"spmv_count": int(out[0]), "spmv_ms": float(out[1]), "prec_count": int(out[2]),
                 "prec_ms": float(out[3])}
 
+    def time_line_solve(self, applications: int = 20) -> dict:
+        """Time the line Gauss-Seidel solve kernel launch by launch (HIP
+        events on the model's stream) over `applications` preconditioner
+        applications without graph replay, on the setup of the last linear
+        solve; returns the launches timed, their total ms and their
+        algorithmic bytes."""
+        o = self.linear_solver.options()
+        o.use_graphs = 0
+        n = 2 * self.nc + 3 * self.nw
+        with torch.cuda.device(self.device):
+            gen = torch.Generator(device=self.device).manual_seed(0)
+            r = torch.randn(n, dtype=torch.float64, device=self.device, generator=gen)
+            z = torch.empty_like(r)
+            _lib.check(self.lib.pmx_ow_profile(self.h, 1), "pmx_ow_profile")
+            out = np.zeros(3)
+            try:
+                for _ in range(applications):
+                    _lib.check(self.lib.pmx_ow_precondition(self.h, ctypes.byref(o), _lib.ptr(r), _lib.ptr(z)),
+                               "pmx_ow_precondition")
+                _lib.check(self.lib.pmx_ow_profile_line_gs(self.h, _lib.ptr(out)), "pmx_ow_profile_line_gs")
+            finally:
+                self.lib.pmx_ow_profile(self.h, 0)
+        return {"launches": int(out[0]), "ms": float(out[1]), "bytes": float(out[2])}
+
     @property
     def nnzb(self) -> int:
         v = ctypes.c_int32()
