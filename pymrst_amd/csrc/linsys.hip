@@ -1192,6 +1192,282 @@ __global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __re
   }
 }
 
+// Twisted ("burn at both ends") block factorization of each line: the top
+// half [0, m) is factored downwards (D'_k = D_k - L_k D'_{k-1}^{-1} U_{k-1}),
+// the bottom half (m, L) upwards (D''_k = D_k - U_k D''_{k+1}^{-1} L_{k+1}),
+// m = L / 2, and the middle pivot takes both couplings.  At every position
+// `Lo` holds the coupling towards the previous cell of that half's walk and
+// `Up` the coupling towards the next one back (L_k, U_k on top; U_k, L_k
+// below), so both halves are solved by the same walk: forward from the line's
+// end to the middle, then back out.
+template <int B, class T>
+__global__ void k_line_factor_tw(int nl, const int* __restrict__ len, const int* __restrict__ cell,
+                                 const int* __restrict__ lo, const int* __restrict__ up,
+                                 const int* __restrict__ diag, const double* __restrict__ val,
+                                 T* __restrict__ inv, T* __restrict__ Lo, T* __restrict__ Up) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  const int L = len[l];
+  if (L == 0) return;
+  const int m = L / 2;
+  double zero[B][B];
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) zero[i][j] = 0.0;
+  // D - A P C for the walk step (A: coupling to the previous cell, C: that
+  // cell's coupling back)
+  auto schur = [&](double (&D)[B][B], const double (&A)[B][B], const double (&P)[B][B], const double (&Cb)[B][B]) {
+    double t[B][B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += A[i][q] * P[q][j];
+        t[i][j] = v;
+      }
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += t[i][q] * Cb[q][j];
+        D[i][j] -= v;
+      }
+  };
+  double Ptop[B][B], Utop[B][B], Pbot[B][B], Lbot[B][B];
+  for (int k = 0; k < m; ++k) {
+    const size_t at = (size_t)k * nl + l;
+    double D[B][B];
+    blk_load<B>(val + (size_t)diag[cell[at]] * B * B, D);
+    if (k > 0) {
+      double A[B][B];
+      blk_load<B>(val + (size_t)lo[at] * B * B, A);
+      schur(D, A, Ptop, Utop);
+      blk_store<B, T>(Lo + at * B * B, A);
+    } else {
+      blk_store<B, T>(Lo + at * B * B, zero);
+    }
+    small_inverse<B>(D, Ptop);
+    blk_store<B, T>(inv + at * B * B, Ptop);
+    blk_load<B>(val + (size_t)up[at] * B * B, Utop);      // k < m <= L - 1: a next cell exists
+    blk_store<B, T>(Up + at * B * B, Utop);
+  }
+  for (int k = L - 1; k > m; --k) {
+    const size_t at = (size_t)k * nl + l;
+    double D[B][B];
+    blk_load<B>(val + (size_t)diag[cell[at]] * B * B, D);
+    if (k < L - 1) {
+      double A[B][B];
+      blk_load<B>(val + (size_t)up[at] * B * B, A);
+      schur(D, A, Pbot, Lbot);
+      blk_store<B, T>(Lo + at * B * B, A);
+    } else {
+      blk_store<B, T>(Lo + at * B * B, zero);
+    }
+    small_inverse<B>(D, Pbot);
+    blk_store<B, T>(inv + at * B * B, Pbot);
+    blk_load<B>(val + (size_t)lo[at] * B * B, Lbot);      // k > m >= 0: a previous cell exists
+    blk_store<B, T>(Up + at * B * B, Lbot);
+  }
+  const size_t at = (size_t)m * nl + l;
+  double D[B][B], Di[B][B];
+  blk_load<B>(val + (size_t)diag[cell[at]] * B * B, D);
+  if (m > 0) {
+    double A[B][B];
+    blk_load<B>(val + (size_t)lo[at] * B * B, A);
+    schur(D, A, Ptop, Utop);
+    blk_store<B, T>(Lo + at * B * B, A);
+  } else {
+    blk_store<B, T>(Lo + at * B * B, zero);
+  }
+  if (m < L - 1) {
+    double A[B][B];
+    blk_load<B>(val + (size_t)up[at] * B * B, A);
+    schur(D, A, Pbot, Lbot);
+    blk_store<B, T>(Up + at * B * B, A);
+  } else {
+    blk_store<B, T>(Up + at * B * B, zero);
+  }
+  small_inverse<B>(D, Di);
+  blk_store<B, T>(inv + at * B * B, Di);
+}
+
+// One colour's line solves with the twisted factors: a wave holds 32 lines;
+// lanes 0-31 walk their lines' top halves downwards, lanes 32-63 the bottom
+// halves upwards (the same code, positions mirrored), the two lanes of a
+// line meet at its middle cell through a lane exchange, and both walk back
+// out.  Every load is at a position inside its line (steps past a lane's
+// walk are clamped and their results dropped), so lines of a wave may differ
+// in length.
+template <int B, class T, int D>
+__global__ __launch_bounds__(64) void k_line_gs_tw(int nl, int lc0, const int* __restrict__ len,
+                                                   const int* __restrict__ cell, const T* __restrict__ inv,
+                                                   const T* __restrict__ Lo, const T* __restrict__ Up,
+                                                   T* __restrict__ r2l, const T* __restrict__ xpl,
+                                                   const PrecIO* __restrict__ io, T* __restrict__ ycell,
+                                                   const double* __restrict__ pcw) {
+  constexpr int BB = B * B;
+  double* __restrict__ z = io->z;
+  double pc[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) pc[a] = pcw[a];
+  const int h = threadIdx.x >> 5;                       // 0: top half, 1: bottom half
+  const int l = lc0 + blockIdx.x * 32 + (threadIdx.x & 31);
+  const int Lw = len[lc0 + blockIdx.x * 32];            // the wave's longest line (lines sorted by length)
+  if (Lw == 0) return;                                  // a wave of padding lines only (wave-uniform)
+  const int Ll = len[l];
+  const int m = Ll / 2;
+  const int J = Ll == 0 ? 0 : (h == 0 ? m : Ll - 1 - m);  // walk steps of this lane
+  const int Jmax = (Lw / 2 + D - 1) / D * D;            // the longest walk of the wave, padded
+  auto pos = [&](int j) -> int {
+    const int jj = min(max(j, 0), max(J - 1, 0));
+    return Ll == 0 ? 0 : (h == 0 ? jj : Ll - 1 - jj);
+  };
+  T ri[D][BB], rl[D][BB], rr[D][B];
+  auto fload = [&](int j, T (&vi)[BB], T (&vl)[BB], T (&vr)[B]) {
+    const size_t at = (size_t)pos(j) * nl + l;
+#pragma unroll
+    for (int q = 0; q < BB; ++q) {
+      vi[q] = inv[at * BB + q];
+      vl[q] = Lo[at * BB + q];
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) vr[a] = r2l[at * B + a];
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) fload(u, ri[u], rl[u], rr[u]);
+  double y[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) y[a] = 0.0;
+  for (int j0 = 0; j0 < Jmax; j0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 + u;
+      double rhs[B], yn[B];
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = (double)rr[u][a];
+#pragma unroll
+        for (int q = 0; q < B; ++q) v -= (double)rl[u][a * B + q] * y[q];
+        rhs[a] = v;
+      }
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += (double)ri[u][a * B + q] * rhs[q];
+        yn[a] = v;
+      }
+      if (j < J) {
+        const size_t at = (size_t)pos(j) * nl + l;
+#pragma unroll
+        for (int a = 0; a < B; ++a) {
+          y[a] = yn[a];
+          r2l[at * B + a] = (T)yn[a];
+        }
+      }
+      fload(j + D, ri[u], rl[u], rr[u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // the middle cell: its pivot takes the top walk's last y through Lo and the
+  // bottom walk's through Up
+  double yo[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) yo[a] = __shfl_xor(y[a], 32);
+  double xm[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) xm[a] = 0.0;
+  if (Ll > 0) {
+    const size_t at = (size_t)m * nl + l;
+    double rhs[B];
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+      double v = (double)r2l[at * B + a];
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const double yt = h == 0 ? y[q] : yo[q], yb = h == 0 ? yo[q] : y[q];
+        v -= (double)Lo[at * BB + a * B + q] * yt + (double)Up[at * BB + a * B + q] * yb;
+      }
+      rhs[a] = v;
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < B; ++q) v += (double)inv[at * BB + a * B + q] * rhs[q];
+      xm[a] = v;
+    }
+    const int c = cell[at];
+    if (h == 0 && c >= 0) {
+      const double xp = (double)xpl[at];
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        z[(size_t)c * B + a] = xm[a] + pc[a] * xp;
+        if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xm[a];
+      }
+    }
+  }
+  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
+  int bc[D];
+  auto bload = [&](int j, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
+    const size_t at = (size_t)pos(j) * nl + l;
+#pragma unroll
+    for (int q = 0; q < BB; ++q) {
+      vi[q] = inv[at * BB + q];
+      vu[q] = Up[at * BB + q];
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
+    vx = xpl[at];
+    c = cell[at];
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) bload(Jmax - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
+  double xn[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) xn[a] = xm[a];
+  for (int j0 = Jmax - 1; j0 >= 0; j0 -= D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 - u;
+      double tq[B], xv[B];
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < B; ++q) v += (double)bu[u][a * B + q] * xn[q];
+        tq[a] = v;
+      }
+#pragma unroll
+      for (int a = 0; a < B; ++a) {
+        double v = (double)by[u][a];
+#pragma unroll
+        for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
+        xv[a] = v;
+      }
+      if (j < J) {
+#pragma unroll
+        for (int a = 0; a < B; ++a) xn[a] = xv[a];
+        const int c = bc[u];
+        if (c >= 0) {
+#pragma unroll
+          for (int a = 0; a < B; ++a) {
+            z[(size_t)c * B + a] = xv[a] + pc[a] * (double)bx[u];
+            if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xv[a];
+          }
+        }
+      }
+      bload(j - D, bi[u], bu[u], by[u], bx[u], bc[u]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Aggregation AMG kernels (scalar CSR, value type T)
 // ---------------------------------------------------------------------------
@@ -1768,7 +2044,7 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::lf_up(*C).zero(s);
     V::r2(*C).zero(s);
     V::xpl(*C).zero(s);
-    hipLaunchKernelGGL((k_line_factor<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
+    hipLaunchKernelGGL((k_line_factor_tw<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
                        C->ln_len.p, C->ln_cell.p, C->ln_lo.p, C->ln_up.p, S.diag.p, S.val.p, V::lf_inv(*C).p,
                        V::lf_lo(*C).p, V::lf_up(*C).p);
     launch_check();
@@ -2004,12 +2280,12 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
         hipEvent_t e0 = nullptr, e1 = nullptr;
         const bool timed = !C->capturing && C->lgs_t.pair(&e0, &e1);
         if (timed)
-          hipExtLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, e0, e1, 0,
+          hipExtLaunchKernelGGL((k_line_gs_tw<B, T, D>), dim3(nlc / 32), dim3(64), 0, s, e0, e1, 0,
                                 C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
                                 V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p,
                                 C->pcdev);
         else
-          hipLaunchKernelGGL((k_line_gs<B, T, D>), dim3(nlc / LINE_GROUP), dim3(LINE_GROUP), 0, s, C->nlines, lc0,
+          hipLaunchKernelGGL((k_line_gs_tw<B, T, D>), dim3(nlc / 32), dim3(64), 0, s, C->nlines, lc0,
                              C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
                              V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
         launch_check();
