@@ -852,57 +852,6 @@ __device__ __forceinline__ void blk_store(T* p, const double a[B][B]) {
     for (int j = 0; j < B; ++j) p[i * B + j] = (T)a[i][j];
 }
 
-// Block LU along each line: D'_k = D_k - L_k D'_{k-1}^{-1} U_{k-1}.
-template <int B, class T>
-__global__ void k_line_factor(int nl, const int* __restrict__ len, const int* __restrict__ cell,
-                              const int* __restrict__ lo, const int* __restrict__ up, const int* __restrict__ diag,
-                              const double* __restrict__ val, T* __restrict__ inv, T* __restrict__ Lo,
-                              T* __restrict__ Up) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nl) return;
-  const int L = len[l];
-  double Pinv[B][B], Uprev[B][B];
-  for (int k = 0; k < L; ++k) {
-    const size_t at = (size_t)k * nl + l;
-    const int c = cell[at];
-    double D[B][B], Di[B][B];
-    blk_load<B>(val + (size_t)diag[c] * B * B, D);
-    if (k > 0) {
-      double Lk[B][B], t[B][B];
-      blk_load<B>(val + (size_t)lo[at] * B * B, Lk);
-#pragma unroll
-      for (int i = 0; i < B; ++i)
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-          double s = 0.0;
-#pragma unroll
-          for (int q = 0; q < B; ++q) s += Lk[i][q] * Pinv[q][j];
-          t[i][j] = s;
-        }
-#pragma unroll
-      for (int i = 0; i < B; ++i)
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-          double s = 0.0;
-#pragma unroll
-          for (int q = 0; q < B; ++q) s += t[i][q] * Uprev[q][j];
-          D[i][j] -= s;
-        }
-      blk_store<B, T>(Lo + at * B * B, Lk);
-    }
-    small_inverse<B>(D, Di);
-    blk_store<B, T>(inv + at * B * B, Di);
-#pragma unroll
-    for (int i = 0; i < B; ++i)
-#pragma unroll
-      for (int j = 0; j < B; ++j) Pinv[i][j] = Di[i][j];
-    if (k + 1 < L) {
-      blk_load<B>(val + (size_t)up[at] * B * B, Uprev);
-      blk_store<B, T>(Up + at * B * B, Uprev);
-    }
-  }
-}
-
 // Line Gauss-Seidel local stage.  The local system is A y = r1 on the cell
 // rows, r1 = r - J P xp - sum Um_j hp_w (the pressure correction and the
 // wells' mixture coupling of P xp removed).  Lines of one colour share no
@@ -1064,132 +1013,17 @@ __global__ void k_gather_off(size_t ne, const int* __restrict__ off_q, const dou
   for (int m = 0; m < B * B; ++m) aoff[e * B * B + m] = q >= 0 ? (T)val[(size_t)q * B * B + m] : (T)0;
 }
 
-// Block-tridiagonal solves of the lines of one colour, one thread per line:
-// forward y_k = D'_k^{-1} (r2_k - L_k y_{k-1}) (y overwrites r2 in place),
-// backward x_k = y_k - D'_k^{-1} U_k x_{k+1}; then z = P xp + x and, when
-// ycell is given, y_cell = x for the colours and sweeps that follow.  (The
-// same walk with D'^{-1} U premultiplied and z written in the last sweep
-// only reads and writes fewer bytes but measured 7% slower per launch.)  A wave
-// walks the padded length of its longest line (lines are sorted by length,
-// tables are zero past each line's end and every colour fills whole waves),
-// so every load address is known without a data-dependent index and the
-// next D positions are in flight while the recurrence runs; the scheduling
-// barriers keep the compiler from sinking those loads next to their use.
+// Prefetch depth of the line walks below: the next D positions of a walk
+// are in flight while its recurrence runs (every load address is known
+// without a data-dependent index), and the scheduling barriers keep the
+// compiler from sinking those loads next to their use.  One lane per whole
+// line (a single LU walk down and back) took 29.9 us per colour at 60x220x85
+// against 22.4 us for the twisted form; premultiplying D'^{-1} U to move fewer
+// bytes was 7% slower still: the walks are bound by their dependent chains
+// and the bytes in flight, not by bandwidth.
 template <int B, class T>
 constexpr int line_depth() {
   return B * B * (int)sizeof(T) <= 16 ? 16 : (B * B * (int)sizeof(T) <= 36 ? 8 : 4);
-}
-
-template <int B, class T, int D>
-__global__ __launch_bounds__(64) void k_line_gs(int nl, int lc0, const int* __restrict__ len,
-                                                const int* __restrict__ cell, const T* __restrict__ inv,
-                                                const T* __restrict__ Lo, const T* __restrict__ Up,
-                                                T* __restrict__ r2l, const T* __restrict__ xpl,
-                                                const PrecIO* __restrict__ io, T* __restrict__ ycell,
-                                                const double* __restrict__ pcw) {
-  static_assert(LINE_PAD % D == 0, "line tables are padded to a multiple of the prefetch depth");
-  constexpr int BB = B * B;
-  double* __restrict__ z = io->z;
-  double pc[B];
-#pragma unroll
-  for (int a = 0; a < B; ++a) pc[a] = pcw[a];
-  const int l = lc0 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int Lp = (len[lc0 + blockIdx.x * blockDim.x] + D - 1) / D * D;
-  // a wave of padding lines only has no positions (the prefetches below
-  // assume Lp >= D); with whole line groups per wave it does not occur
-  if (Lp == 0) return;
-  T ri[D][BB], rl[D][BB], rr[D][B];
-  auto fload = [&](int k, T (&vi)[BB], T (&vl)[BB], T (&vr)[B]) {
-    const size_t at = (size_t)k * nl + l;
-#pragma unroll
-    for (int q = 0; q < BB; ++q) {
-      vi[q] = inv[at * BB + q];
-      vl[q] = Lo[at * BB + q];
-    }
-#pragma unroll
-    for (int a = 0; a < B; ++a) vr[a] = r2l[at * B + a];
-  };
-#pragma unroll
-  for (int u = 0; u < D; ++u) fload(u, ri[u], rl[u], rr[u]);
-  double y[B];
-#pragma unroll
-  for (int a = 0; a < B; ++a) y[a] = 0.0;
-  for (int k0 = 0; k0 < Lp; k0 += D) {
-#pragma unroll
-    for (int u = 0; u < D; ++u) {
-      const int k = k0 + u;
-      double rhs[B];
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double v = (double)rr[u][a];
-#pragma unroll
-        for (int q = 0; q < B; ++q) v -= (double)rl[u][a * B + q] * y[q];
-        rhs[a] = v;
-      }
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double v = 0.0;
-#pragma unroll
-        for (int q = 0; q < B; ++q) v += (double)ri[u][a * B + q] * rhs[q];
-        y[a] = v;
-      }
-      const size_t at = (size_t)k * nl + l;
-#pragma unroll
-      for (int a = 0; a < B; ++a) r2l[at * B + a] = (T)y[a];
-      fload(min(k + D, Lp - 1), ri[u], rl[u], rr[u]);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
-  T bi[D][BB], bu[D][BB], by[D][B], bx[D];
-  int bc[D];
-  auto bload = [&](int k, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
-    const size_t at = (size_t)k * nl + l;
-#pragma unroll
-    for (int q = 0; q < BB; ++q) {
-      vi[q] = inv[at * BB + q];
-      vu[q] = Up[at * BB + q];
-    }
-#pragma unroll
-    for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
-    vx = xpl[at];
-    c = cell[at];
-  };
-#pragma unroll
-  for (int u = 0; u < D; ++u) bload(Lp - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
-  double xn[B];
-#pragma unroll
-  for (int a = 0; a < B; ++a) xn[a] = 0.0;
-  for (int k0 = Lp - 1; k0 >= 0; k0 -= D) {
-#pragma unroll
-    for (int u = 0; u < D; ++u) {
-      const int k = k0 - u;
-      double tq[B];
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double v = 0.0;
-#pragma unroll
-        for (int q = 0; q < B; ++q) v += (double)bu[u][a * B + q] * xn[q];
-        tq[a] = v;
-      }
-#pragma unroll
-      for (int a = 0; a < B; ++a) {
-        double v = (double)by[u][a];
-#pragma unroll
-        for (int q = 0; q < B; ++q) v -= (double)bi[u][a * B + q] * tq[q];
-        xn[a] = v;
-      }
-      const int c = bc[u];
-      if (c >= 0) {
-#pragma unroll
-        for (int a = 0; a < B; ++a) {
-          z[(size_t)c * B + a] = xn[a] + pc[a] * (double)bx[u];
-          if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xn[a];
-        }
-      }
-      bload(max(k - D, 0), bi[u], bu[u], by[u], bx[u], bc[u]);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-  }
 }
 
 // Twisted ("burn at both ends") block factorization of each line: the top
