@@ -30,8 +30,9 @@ def equilibrate(a, sweeps=3):
 
 
 class CPRSolverCPU:
-    def __init__(self, rtol=1e-8, maxiter=200, restart=50):
+    def __init__(self, rtol=1e-8, maxiter=200, restart=50, use_pressure_stage=True):
         self.rtol, self.maxiter, self.restart = rtol, maxiter, restart
+        self.use_pressure_stage = use_pressure_stage
         self.last_iterations = 0
 
     def _weighting(self, model, state, n):
@@ -88,6 +89,8 @@ class CPRSolverCPU:
         local = lambda z: dc * local_solve((dr * z)[perm])[inv_perm]
 
         def precondition(r):
+            if not self.use_pressure_stage:    # the local stage alone (:487-489)
+                return local(r)
             x = prolong @ solve_p(w @ r)
             return x + local(r - a @ x)
 

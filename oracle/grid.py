@@ -216,8 +216,8 @@ def well_index(g: CartGrid, rock: Rock, cells, radius=0.1, skin=0.0, well_dir="z
 
 def add_well(wells: Optional[List[Well]], g: CartGrid, rock: Rock, cells, well_type="bhp",
              val=0.0, radius=0.1, compi=None, sign=0, gravity=None, name=None,
-             ref_depth=None) -> List[Well]:
-    """pymrst/params/wells_and_bc/add_well.py:16-126 (z-direction wells).
+             ref_depth=None, well_dir="z") -> List[Well]:
+    """pymrst/params/wells_and_bc/add_well.py:16-126 (well_dir x, y or z).
 
     dZ = depth - ref_depth where depth is the cell centroid along the
     gravity direction (z when gravity is off) and ref_depth is the shallowest
@@ -225,7 +225,7 @@ def add_well(wells: Optional[List[Well]], g: CartGrid, rock: Rock, cells, well_t
     """
     wells = list(wells or [])
     cells = np.asarray(cells, dtype=np.int64).ravel()
-    wi = well_index(g, rock, cells, radius)
+    wi = well_index(g, rock, cells, radius, well_dir=well_dir)
     compi = np.array([1.0, 0.0, 0.0]) if compi is None else np.asarray(compi, float).ravel()
     depth = g.cell_centroids[cells, 2]
     if ref_depth is None:

@@ -92,6 +92,29 @@ class Ops:
         return np.asarray(x)[up]
 
 
+def segment_mix_fractions(compi, cq_w, cq_o):
+    """_segment_mix_fractions (two_phase_oil_water_model.py:188-206): the
+    surface composition of the flow past each segment of a top-to-bottom
+    perforation chain, i.e. of everything perforated at or below it (a
+    reverse cumulative sum); segments with zero net flow take compi."""
+    seg_w = np.cumsum(np.asarray(cq_w, float)[::-1])[::-1]
+    seg_o = np.cumsum(np.asarray(cq_o, float)[::-1])[::-1]
+    seg_t = seg_w + seg_o
+    dead = seg_t == 0.0
+    safe = np.where(dead, 1.0, seg_t)
+    return np.where(dead, compi[0], seg_w / safe), np.where(dead, compi[1], seg_o / safe)
+
+
+def topology_cdp(dZ, mix_w, mix_o, b_w_c, b_o_c, rhoS, gravity):
+    """_topology_cdp (two_phase_oil_water_model.py:208-224): per-perforation
+    mixture density sum(mix rhoS) / sum(mix / b), and the connection pressure
+    drop cumsum(g rho_mix dz) down the chain from the reference depth."""
+    vr = mix_w / b_w_c + mix_o / b_o_c
+    rho_mix = (mix_w * rhoS[0] + mix_o * rhoS[1]) / vr
+    dz = np.diff(np.concatenate(([0.0], np.asarray(dZ, float))))
+    return np.cumsum(gravity * rho_mix * dz)
+
+
 class OWModel:
     """TwoPhaseOilWaterModelAD restated (two_phase_oil_water_model.py:79-657)."""
 
@@ -195,19 +218,9 @@ class OWModel:
         out, off = [], 0
         for w in self.wells:
             n = w.cells.size
-            qw, qo = cq_sw[off:off + n], cq_so[off:off + n]
+            mix_w, mix_o = segment_mix_fractions(w.compi, cq_sw[off:off + n], cq_so[off:off + n])
             off += n
-            sw_ = np.cumsum(qw[::-1])[::-1]
-            so_ = np.cumsum(qo[::-1])[::-1]
-            st = sw_ + so_
-            dead = st == 0.0
-            safe = np.where(dead, 1.0, st)
-            mw = np.where(dead, w.compi[0], sw_ / safe)
-            mo = np.where(dead, w.compi[1], so_ / safe)
-            vr = mw / b_w0[w.cells] + mo / b_o0[w.cells]
-            rho_mix = (mw * f.rhoS[0] + mo * f.rhoS[1]) / vr
-            dz = np.diff(np.concatenate(([0.0], w.dZ)))
-            out.append(np.cumsum(self.gravity * rho_mix * dz))
+            out.append(topology_cdp(w.dZ, mix_w, mix_o, b_w0[w.cells], b_o0[w.cells], f.rhoS, self.gravity))
         self._last_cdp = out
         return out
 

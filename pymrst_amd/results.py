@@ -152,208 +152,264 @@ def _time_of(state: Any) -> float:
     return float(t)
 
 
+class _MemoryShelf:
+    """Steps held as Python objects, keyed by id."""
+
+    def __init__(self):
+        self.items: Dict[int, Any] = {}
+
+    def ids(self) -> List[int]:
+        return sorted(self.items)
+
+    def has(self, index: int) -> bool:
+        return index in self.items
+
+    def load(self, index: int) -> Any:
+        return self.items[index]
+
+    def save(self, index: int, value: Any) -> None:
+        self.items[index] = value
+
+    def remove(self, index: int) -> None:
+        self.items.pop(index, None)
+
+    def remove_all(self) -> None:
+        self.items.clear()
+
+
+class _DiskShelf:
+    """One ``<prefix><id>.npz`` file per step in one directory.  Files are
+    written beside their final name and renamed into place, so a run killed
+    while writing never leaves a torn step; reads are confined to the
+    directory."""
+
+    def __init__(self, folder: Path, prefix: str, suffix: str, verbose: bool):
+        self.folder, self.prefix, self.suffix, self.verbose = folder, prefix, suffix, verbose
+
+    def file(self, index: int) -> Path:
+        return self.folder / f"{self.prefix}{index}{self.suffix}"
+
+    def _id_of(self, name: str) -> Optional[int]:
+        if not (name.startswith(self.prefix) and name.endswith(self.suffix)):
+            return None
+        digits = name[len(self.prefix):len(name) - len(self.suffix)]
+        return int(digits) if digits.isdigit() else None
+
+    def ids(self) -> List[int]:
+        if not self.folder.is_dir():
+            return []
+        found = (self._id_of(entry.name) for entry in self.folder.iterdir())
+        return sorted(i for i in found if i is not None)
+
+    def has(self, index: int) -> bool:
+        return self.file(index).is_file()
+
+    def _checked(self, index: int) -> Path:
+        target = self.file(index).resolve()
+        home = self.folder.resolve()
+        if home != target.parent and home not in target.parents:
+            raise ValueError(f"refusing to read {target}: outside the store directory {home}")
+        return target
+
+    def load(self, index: int) -> Any:
+        target = self._checked(index)
+        if not target.is_file():
+            raise KeyError(f"step {index} is not stored in {self.folder}")
+        with np.load(target, allow_pickle=False) as npz:
+            return decode({name: npz[name] for name in npz.files})
+
+    def save(self, index: int, value: Any) -> None:
+        self.folder.mkdir(parents=True, exist_ok=True)
+        final = self.file(index)
+        staging = final.parent / (final.name + ".part")
+        with open(staging, "wb") as handle:
+            np.savez(handle, **encode(value))
+        staging.replace(final)
+        if self.verbose:
+            print(f"stored step {index} in {final}")
+
+    def remove(self, index: int) -> None:
+        target = self.file(index)
+        if target.is_file():
+            target.unlink()
+            if self.verbose:
+                print(f"removed step {index} ({target})")
+
+    def remove_all(self) -> None:
+        for index in self.ids():
+            self.remove(index)
+
+
+def _camel(name: str) -> str:
+    head, *rest = name.split("_")
+    return head + "".join(word.capitalize() for word in rest)
+
+
 class ResultHandler:
-    """A list-like store of per-step results, in memory and/or on disk
-    (result_handler.py:42-318); steps on disk are ``.npz`` files of plain
-    arrays."""
+    """A store of per-step results addressed by MRST's 1-based step ids,
+    held in memory, on disk, or both (the interface of the reference's
+    ResultHandler, pymrst/modules/ad_core/utils/result_handler.py:42-318).
+
+    The two places are independent shelves: a write goes to every enabled
+    shelf, a read prefers memory, and the ids a store reports are those of
+    its durable shelf (disk when enabled, else memory), gaps included.  A
+    step on disk is one ``.npz`` of plain arrays (:func:`encode`)."""
 
     def __init__(self, data_directory: Optional[str | Path] = None, data_folder: str = "results",
                  data_prefix: str = "state", write_to_disk: bool = False, store_in_memory: bool = True,
                  extension: str = ".npz", cleardir: bool = False, verbose: bool = False):
         if extension != ".npz":
-            raise ValueError("steps are stored as .npz files of plain arrays")
+            raise ValueError(f"extension {extension!r} is not supported: steps are .npz files of plain arrays")
         self.data_directory = Path(data_directory) if data_directory else Path.cwd()
-        self.data_folder = data_folder
-        self.data_prefix = data_prefix
-        self.write_to_disk = bool(write_to_disk)
-        self.store_in_memory = bool(store_in_memory)
-        self.extension = extension
+        self.data_folder, self.data_prefix, self.extension = data_folder, data_prefix, extension
+        self.write_to_disk, self.store_in_memory = bool(write_to_disk), bool(store_in_memory)
         self.verbose = bool(verbose)
-        self.data: Dict[int, Any] = {}
+        self._memory = _MemoryShelf()
+        self._disk = _DiskShelf(self.data_directory / data_folder, data_prefix, extension, self.verbose)
         if self.write_to_disk:
-            self.get_data_path().mkdir(parents=True, exist_ok=True)
+            self._disk.folder.mkdir(parents=True, exist_ok=True)
             if cleardir:
-                for path in self.get_data_path().glob(f"{self.data_prefix}*{self.extension}"):
-                    path.unlink()
+                self._disk.remove_all()
 
-    # paths
+    @property
+    def data(self) -> Dict[int, Any]:
+        """The steps held in memory."""
+        return self._memory.items
+
+    def _shelves(self):
+        return ([self._memory] if self.store_in_memory else []) + ([self._disk] if self.write_to_disk else [])
+
+    def _durable(self):
+        return self._disk if self.write_to_disk else self._memory
+
+    # -- locations ---------------------------------------------------------
     def get_extension(self, wildcard: bool = False) -> str:
-        return f"*{self.extension}" if wildcard else self.extension
+        return ("*" if wildcard else "") + self.extension
 
     def get_data_path(self, index: Optional[int] = None) -> Path:
-        base = self.data_directory / self.data_folder
         if index is None:
-            return base
-        if not isinstance(index, (int, np.integer)) or isinstance(index, bool):
-            raise TypeError(f"index must be a single integer, got {index!r}")
-        return base / f"{self.data_prefix}{int(index)}{self.extension}"
+            return self._disk.folder
+        if isinstance(index, bool) or not isinstance(index, (int, np.integer)):
+            raise TypeError(f"a step id is one integer, not {index!r}")
+        return self._disk.file(int(index))
 
-    # disk
+    # -- files -------------------------------------------------------------
     def write_to_file(self, data: Any, index: int) -> "ResultHandler":
-        path = self.get_data_path(index)
-        path.parent.mkdir(parents=True, exist_ok=True)
-        arrays = encode(data)
-        # write beside, then rename: a run killed mid-write leaves no torn step
-        tmp = path.with_name(path.name + ".part")
-        with tmp.open("wb") as stream:
-            np.savez(stream, **arrays)
-        tmp.replace(path)
-        if self.verbose:
-            print(f"writing data to {path}")
+        self._disk.save(int(index), data)
         return self
 
     def read_from_file(self, indices: int | Sequence[int]) -> List[Any]:
-        if isinstance(indices, (int, np.integer)):
-            indices = [indices]
-        out = []
-        root = self.get_data_path().resolve()
-        for index in indices:
-            path = self.get_data_path(index).resolve()
-            if root not in path.parents:
-                raise ValueError(f"{path} is outside the handler's directory {root}")
-            if not path.exists():
-                raise KeyError(f"no stored result for index {index}")
-            with np.load(path, allow_pickle=False) as z:
-                out.append(decode({k: z[k] for k in z.files}))
-        return out
+        wanted = [indices] if isinstance(indices, (int, np.integer)) else list(indices)
+        return [self._disk.load(int(i)) for i in wanted]
 
     def delete_file(self, index: int) -> "ResultHandler":
-        path = self.get_data_path(index)
-        if path.exists():
-            path.unlink()
-            if self.verbose:
-                print(f"deleting data at {path}")
+        self._disk.remove(int(index))
         return self
 
-    # what is stored
+    # -- what is held ------------------------------------------------------
     def get_valid_ids(self) -> List[int]:
-        if self.write_to_disk:
-            pattern = re.compile(rf"^{re.escape(self.data_prefix)}(\d+){re.escape(self.extension)}$")
-            directory = self.get_data_path()
-            if not directory.is_dir():
-                return []
-            return sorted(int(m.group(1)) for m in (pattern.match(p.name) for p in directory.iterdir()) if m)
-        return sorted(self.data)
+        return self._durable().ids()
 
     def numel_data(self) -> int:
-        ids = self.get_valid_ids()
-        return max(ids) if ids else 0
+        held = self.get_valid_ids()
+        return held[-1] if held else 0
 
     def size_data(self, dim: Optional[int] = None):
         shape = (self.numel_data(), 1)
-        return shape if dim is None else shape[dim - 1]
+        return shape[dim - 1] if dim else shape
 
     def __len__(self) -> int:
         return len(self.get_valid_ids())
 
-    def __iter__(self) -> Iterator[Any]:
-        for index in self.get_valid_ids():
-            yield self[index]
+    def __contains__(self, index: int) -> bool:
+        return self._durable().has(index)
 
-    # element access
+    def __iter__(self) -> Iterator[Any]:
+        return (self[i] for i in self.get_valid_ids())
+
+    # -- access ------------------------------------------------------------
+    def __getitem__(self, index: int) -> Any:
+        for shelf in self._shelves():
+            if shelf.has(index):
+                return shelf.load(index)
+        raise KeyError(f"step {index} is not in this store")
+
+    def __setitem__(self, index: int, value: Any) -> None:
+        shelves = self._shelves()
+        if not shelves:
+            raise ValueError("with neither memory nor disk enabled every stored step would be discarded; "
+                             "enable store_in_memory or write_to_disk")
+        if callable(getattr(value, "to_host", None)):
+            value = value.to_host()
+        for shelf in shelves:
+            shelf.save(index, value)
+
+    def __delitem__(self, index: int) -> None:
+        self._memory.remove(index)
+        if self.write_to_disk:
+            self._disk.remove(index)
+
     def get_data(self, indices: Optional[Sequence[int]] = None) -> List[Any]:
         return [self[i] for i in (self.get_valid_ids() if indices is None else indices)]
 
-    def __getitem__(self, index: int) -> Any:
-        if self.store_in_memory and index in self.data:
-            return self.data[index]
-        if self.write_to_disk:
-            return self.read_from_file(index)[0]
-        raise KeyError(f"no stored result for index {index}")
-
-    def __setitem__(self, index: int, value: Any) -> None:
-        if not self.store_in_memory and not self.write_to_disk:
-            raise ValueError("this handler neither stores in memory nor writes to disk, so the value "
-                             "would be discarded; set store_in_memory or write_to_disk")
-        if hasattr(value, "to_host") and callable(value.to_host):
-            value = value.to_host()
-        if self.store_in_memory:
-            self.data[index] = value
-        if self.write_to_disk:
-            self.write_to_file(value, index)
-
-    def __delitem__(self, index: int) -> None:
-        self.data.pop(index, None)
-        if self.write_to_disk:
-            self.delete_file(index)
-
-    def __contains__(self, index: int) -> bool:
-        return index in self.get_valid_ids()
-
     def reset(self) -> "ResultHandler":
-        for index in list(self.get_valid_ids()):
-            del self[index]
-        self.data.clear()
+        self.reset_data()
         return self
 
     def reset_data(self, subs=None) -> None:
-        """Drop everything, or the ids (or the 1-based mask) in `subs`
-        (result_handler.py:221-247)."""
+        """Remove every step, or the steps `subs` names: ids, or a boolean
+        mask over ids 1, 2, ... (result_handler.py:221-247)."""
         if subs is None:
-            self.data.clear()
+            self._memory.remove_all()
             if self.write_to_disk:
-                for path in self.get_data_path().glob(f"{self.data_prefix}{self.get_extension(True)}"):
-                    path.unlink()
+                self._disk.remove_all()
             return
-        subs = np.atleast_1d(np.asarray(subs))
-        if subs.dtype == bool:
-            subs = np.flatnonzero(subs) + 1
-        for index in subs.astype(int).tolist():
-            if self.store_in_memory:
-                self.data.pop(index, None)
-            if self.write_to_disk:
-                self.delete_file(index)
+        sel = np.atleast_1d(np.asarray(subs))
+        doomed = (np.nonzero(sel)[0] + 1) if sel.dtype == bool else sel.astype(np.int64)
+        for index in map(int, doomed):
+            for shelf in self._shelves():
+                shelf.remove(index)
 
-    # searching
+    # -- search ------------------------------------------------------------
+    def _walk(self, reverse: bool):
+        held = self.get_valid_ids()
+        return reversed(held) if reverse else iter(held)
+
     def get_first_by_function(self, predicate: Callable[[Any], bool], reverse: bool = False):
-        ids = self.get_valid_ids()
-        for index in (ids[::-1] if reverse else ids):
-            candidate = self[index]
-            if predicate(candidate):
-                return candidate, index
+        for index in self._walk(reverse):
+            step = self[index]
+            if predicate(step):
+                return step, index
         return None, None
 
     def get_by_time(self, time: float, tol: float = 1e-3, reverse: bool = False):
-        return self.get_first_by_function(lambda s: abs(_time_of(s) - time) < tol, reverse)
+        return self.get_first_by_function(lambda step: abs(_time_of(step) - time) < tol, reverse)
 
     def get_closest_by_time(self, time: float, reverse: bool = False):
-        best = (None, None, float("inf"))
-        ids = self.get_valid_ids()
-        for index in (ids[::-1] if reverse else ids):
-            candidate = self[index]
-            d = abs(_time_of(candidate) - time)
-            if d < best[2]:
-                best = (candidate, index, d)
-        return best
+        scored = [(abs(_time_of(self[i]) - time), n, i) for n, i in enumerate(self._walk(reverse))]
+        if not scored:
+            return None, None, float("inf")
+        gap, _, index = min(scored)
+        return self[index], index, gap
 
     def get_restart(self) -> Tuple[Any, int]:
-        """(last stored state, the step to resume at); (None, 1) when empty."""
-        ids = self.get_valid_ids()
-        if not ids:
-            return None, 1
-        last = max(ids)
-        return self[last], last + 1
+        """(the last stored step, the id to continue at); (None, 1) when the
+        store is empty."""
+        last = self.numel_data()
+        return (None, 1) if last == 0 else (self[last], last + 1)
 
     def __repr__(self) -> str:
-        where = (["memory"] if self.store_in_memory else []) + (
-            [str(self.get_data_path())] if self.write_to_disk else [])
-        return f"ResultHandler({len(self)} result(s) in {' + '.join(where) or 'nothing'})"
+        places = [name for name, on in (("memory", self.store_in_memory),
+                                         (str(self._disk.folder), self.write_to_disk)) if on]
+        return f"ResultHandler({len(self)} step(s) in {' and '.join(places) or 'no place'})"
 
-    # MRST's names
-    getExtension = get_extension
-    getDataPath = get_data_path
-    writeToFile = write_to_file
-    readFromFile = read_from_file
-    deleteFile = delete_file
-    getValidIds = get_valid_ids
-    numelData = numel_data
-    sizeData = size_data
-    getData = get_data
-    resetData = reset_data
-    getFirstByFunction = get_first_by_function
-    getByTime = get_by_time
-    getClosestByTime = get_closest_by_time
-    getRestart = get_restart
+
+# MRST's camel-case method names, as the reference also exposes them
+for _name in ("get_extension", "get_data_path", "write_to_file", "read_from_file", "delete_file", "get_valid_ids",
+              "numel_data", "size_data", "get_data", "reset_data", "get_first_by_function", "get_by_time",
+              "get_closest_by_time", "get_restart"):
+    setattr(ResultHandler, _camel(_name), getattr(ResultHandler, _name))
+del _name
 
 
 def simulate_with_output(state0, model, timesteps: Sequence[float], states: ResultHandler,
@@ -378,26 +434,32 @@ def simulate_with_output(state0, model, timesteps: Sequence[float], states: Resu
     ids = states.get_valid_ids()
     if ids and ids != list(range(1, max(ids) + 1)):
         raise ValueError(f"the store holds steps {ids}, not a contiguous run from step 1")
+    remembers = callable(getattr(model, "run_memory", None))
+    if remembers and reports is not None:
+        # a state whose report (and run memory) never reached the store is
+        # an interrupted write: drop it and run that step again
+        while ids and ids[-1] not in reports:
+            del states[ids.pop()]
     last, start = states.get_restart()
     if start - 1 > len(timesteps):
         raise ValueError(f"the store holds step {start - 1}, beyond the schedule's {len(timesteps)} steps")
     state = state0 if last is None else last
-    remembers = callable(getattr(model, "run_memory", None))
     if remembers and reports is not None and start > 1:
-        if (start - 1) not in reports:
-            raise ValueError(f"the report store lacks step {start - 1}, which the state store holds")
         model.restore_run_memory(reports[start - 1].get("model_memory"))
     its = np.zeros(len(timesteps), dtype=np.int64)
     for i in range(start - 1, len(timesteps)):
         out, nit = simulate(state, model, [float(timesteps[i])], **kwargs)
         state = out[-1]
         its[i] = int(nit[0])
-        states[i + 1] = state
+        # the report (with the run memory) first: a run killed between the
+        # two writes leaves an orphan report, which the next step overwrites,
+        # never a state without its memory
         if reports is not None:
             rep = {"time": float(timesteps[:i + 1].sum()), "dt": float(timesteps[i]), "iterations": int(its[i])}
             if remembers:
                 rep["model_memory"] = model.run_memory()
             reports[i + 1] = rep
+        states[i + 1] = state
     return [states[i] for i in range(1, len(timesteps) + 1)], its
 
 

@@ -185,3 +185,44 @@ def test_interrupted_schedule_resumes_exactly(tmp_path):
     st2[7] = full[0]
     with pytest.raises(ValueError, match="contiguous"):
         simulate_with_output(s0, mo, dts, st2, simulate=oow.simulate_schedule)
+
+
+def test_a_state_without_its_report_is_run_again(tmp_path):
+    """A run killed after writing step 2's state but before its report (the
+    report is written first, so this only arises from an older or foreign
+    store) resumes by dropping that state and running step 2 again: the
+    result equals the uninterrupted run bit for bit."""
+    from pymrst_amd.results import ResultHandler, simulate_with_output
+    inp = small_gravity_inputs()
+    dts = inp["dts"]
+    mo, s0 = build_oracle(inp)
+    full, full_its = oow.simulate_schedule(s0, mo, dts)
+
+    def handlers():
+        return (ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False),
+                ResultHandler(tmp_path, data_prefix="report", write_to_disk=True, store_in_memory=False))
+
+    st, rep = handlers()
+    simulate_with_output(s0, _remembering(build_oracle(inp)[0]), dts[:2], st, rep, simulate=oow.simulate_schedule)
+    rep.get_data_path(2).unlink()
+    st, rep = handlers()
+    states, its = simulate_with_output(s0, _remembering(build_oracle(inp)[0]), dts, st, rep,
+                                       simulate=oow.simulate_schedule)
+    assert its.tolist() == [0] + full_its[1:].tolist()
+    assert st.get_valid_ids() == rep.get_valid_ids() == [1, 2, 3]
+    for a, b in zip(states, full):
+        assert a.pressure.tobytes() == b.pressure.tobytes() and a.sW.tobytes() == b.sW.tobytes()
+
+
+def test_camel_case_names_and_memory_with_disk(tmp_path):
+    """MRST's method names call the same methods; with memory and disk both
+    on, a step is read from memory and the ids are the disk's."""
+    from pymrst_amd.results import ResultHandler
+    h = ResultHandler(tmp_path, write_to_disk=True, store_in_memory=True)
+    h[1] = {"time": 1.0}
+    h.data[1] = {"time": 5.0}            # memory is read first
+    assert h[1]["time"] == 5.0 and h.getValidIds() == h.get_valid_ids() == [1]
+    assert ResultHandler(tmp_path, write_to_disk=True, store_in_memory=False)[1]["time"] == 1.0
+    assert h.getRestart()[1] == 2 and h.numelData() == 1 and h.sizeData(1) == 1
+    del h[1]
+    assert len(h) == 0 and h.data == {}
