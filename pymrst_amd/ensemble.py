@@ -200,10 +200,12 @@ def _transfer(src, s_states, dst, d_states, members, back=False):
         (src if back else dst)._last_cdp = []     # the heads above are the members' current ones
 
 
-def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts, compact, observer, i):
+def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, vol, max_cuts, compact, observer, i):
     """Newton iterations of one union until every member reaches the report
-    time D; per-member arrays are updated in place.  Returns the union
-    iteration count (this level and the levels below)."""
+    time D; per-member arrays are updated in place (vol, (M, 2): the surface
+    water and oil each member's wells have moved, summed over its accepted
+    ministeps as dt times the converged rates).  Returns the union iteration
+    count (this level and the levels below)."""
     M = model.nmem
     done = np.zeros(M, dtype=bool)
     tol = 1e-12
@@ -223,10 +225,10 @@ def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts,
             s0s = sub.device_state(sub.initial_state(model.cases[0].p0, model.cases[0].sw0))
             s1s = s0s.copy()
             _transfer(model, (st0, st), sub, (s0s, s1s), members)
-            sl = [a[members].copy() for a in (t, dtm, cuts, it, its_step, restart)]
+            sl = [a[members].copy() for a in (t, dtm, cuts, it, its_step, restart, vol)]
             n_union += _run_level(sub, s0s, s1s, D, *sl, max_cuts, compact, observer, i)
             _transfer(model, (st,), sub, (s1s,), members, back=True)
-            for a, b in zip((t, dtm, cuts, it, its_step), sl[:5]):
+            for a, b in zip((t, dtm, cuts, it, its_step, vol), sl[:5] + sl[6:]):
                 a[members] = b
             model.stats["newton"] += sub.stats["newton"]
             del sub
@@ -239,7 +241,13 @@ def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts,
         model.assemble_device(st0, st, float(D), cdp)
         _, pmean = model.converged(float(D), st)
         ok = model._ok_m.cpu().numpy()
-        for m in np.flatnonzero(ok & ~done):     # accept the ministep
+        accept = ok & ~done
+        if accept.any():
+            q = torch.zeros((M, 2), dtype=torch.float64, device=model.device)
+            q[:, 0].index_add_(0, model.well_member, st.qws)
+            q[:, 1].index_add_(0, model.well_member, st.qos)
+            vol[accept] += dtm[accept, None] * q.cpu().numpy()[accept]
+        for m in np.flatnonzero(accept):     # accept the ministep
             t[m] += dtm[m]
             its_step[m] += it[m]
             if t[m] >= D * (1.0 - tol):
@@ -280,7 +288,7 @@ def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts,
 
 
 def simulate_ensemble(model: "EnsembleModel", state0, timesteps: Sequence[float], max_cuts: int = 6,
-                      observer=None, keep_device: bool = False, compact: float = 0.3):
+                      observer=None, keep_device: bool = False, compact: float = 0.3, volumes=None):
     """Report steps with per-member time stepping.  Each Newton iteration is
     one union assembly and one (segmented) solve over the members still
     iterating; a member that converges accepts its ministep and starts the
@@ -290,7 +298,10 @@ def simulate_ensemble(model: "EnsembleModel", state0, timesteps: Sequence[float]
     cells, they continue in a union of their own (0 disables), so a few slow
     members do not keep every iteration at the full ensemble's cost.
     Returns (states per report step, union Newton iterations per report
-    step, Newton iterations per report step and member)."""
+    step, Newton iterations per report step and member).  A list given as
+    `volumes` receives, per report step, an (M, 2) array of the surface
+    water and oil each member's wells moved over it (sum over the member's
+    ministeps of dt times its converged well rates; injection positive)."""
     M = model.nmem
     state = model.device_state(state0)
     states, union_its, member_its = [], [], []
@@ -301,8 +312,11 @@ def simulate_ensemble(model: "EnsembleModel", state0, timesteps: Sequence[float]
         t, dtm = np.zeros(M), np.full(M, D)
         cuts, it, its_step = (np.zeros(M, dtype=np.int64) for _ in range(3))
         restart = np.ones(M, dtype=bool)
-        n_union = _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, max_cuts, compact, observer,
-                             i)
+        vol = np.zeros((M, 2))
+        n_union = _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, vol, max_cuts, compact,
+                             observer, i)
+        if volumes is not None:
+            volumes.append(vol)
         state = st
         union_its.append(n_union)
         member_its.append(its_step)
