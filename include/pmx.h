@@ -95,6 +95,17 @@ int pmx_ow_residual(pmx_ow* h, double* out);
  * wells_out (host, 3*nw) = well rows of the residual. */
 int pmx_ow_convergence(pmx_ow* h, const double* p, double* out7, double* wells_out);
 
+/* pmx_ow_convergence without any allocation: the same reductions and
+ * outputs, with the partial sums kept in a static per-GPU array of the
+ * library (512 rows of 7 doubles plus the 7 results, 28,736 bytes, held
+ * until the process ends) instead of handle memory.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two streams, two host
+ * threads) race.  Any cell count fits; a reduction that would need more
+ * partial rows than the array holds is rejected with an error. */
+int pmx_ow_convergence_noalloc(pmx_ow* h, const double* p, double* out7, double* wells_out);
+
 /* Solve J dx = -r (CPR-FGMRES) into dx (device, n = 2*nc + 3*nw); pmean is
  * the mean pressure for the quasi-IMPES weights.  Replaces
  * _solve_increment / CPRSolver.solve (linear_solvers.py:426-545). */

@@ -203,6 +203,14 @@ int pmx_ow_convergence(pmx_ow* h, const double* p, double* out7, double* wells_o
   })
 }
 
+int pmx_ow_convergence_noalloc(pmx_ow* h, const double* p, double* out7, double* wells_out) {
+  PMX_TRY({
+    PMX_CHECK(h && p && out7, "null argument");
+    bind(h);
+    h->m->convergence_noalloc(h->device, p, out7, wells_out);
+  })
+}
+
 int pmx_ow_solve(pmx_ow* h, const pmx_solver_options* o, double pmean, double* dx, pmx_solve_info* info) {
   try {
     PMX_CHECK(h && dx, "null argument");

@@ -22,6 +22,10 @@ class OWModelDev {
   void assemble(const double* p, const double* s, const double* qws, const double* qos, const double* bhp,
                 const double* p0, const double* s0, double dt, const double* cdp_dev);
   void convergence(const double* p, double* out7, double* wells_out);
+  // the same reductions with the partials in a static per-GPU array: no
+  // allocation on any call (device: the GPU this model runs on, current)
+  void convergence_noalloc(int device, const double* p, double* out7, double* wells_out);
+  void reduce_convergence(const double* p, double* out7, double* wells_out, double* partial, int nblk);
   int solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean);
   void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
               double* qos, double* bhp);

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 
 #include "ow.h"
 #include "ow_props.h"
@@ -473,15 +474,55 @@ void OWModelDev::assemble(const double* p, const double* s, const double* qws, c
 }
 
 void OWModelDev::convergence(const double* p, double* out7, double* wells_out) {
-  hipLaunchKernelGGL(k_ow_conv, dim3(kConvBlocks), dim3(256), 0, stream, nc, r.p, p, pv.p, fluid, conv_partial.p);
+  reduce_convergence(p, out7, wells_out, conv_partial.p, kConvBlocks);
+}
+
+// The reductions into `partial` (nblk rows of 7, then the 7 results) and
+// their copy to the host; every partial row and result is written by each
+// call, so the buffer needs no clearing between calls.
+void OWModelDev::reduce_convergence(const double* p, double* out7, double* wells_out, double* partial, int nblk) {
+  hipLaunchKernelGGL(k_ow_conv, dim3(nblk), dim3(256), 0, stream, nc, r.p, p, pv.p, fluid, partial);
   PMX_HIP(hipGetLastError());
-  double* fin = conv_partial.p + (size_t)kConvBlocks * 7;
-  hipLaunchKernelGGL(k_ow_conv_final, dim3(1), dim3(64 * 7), 0, stream, kConvBlocks, conv_partial.p, fin);
+  double* fin = partial + (size_t)nblk * 7;
+  hipLaunchKernelGGL(k_ow_conv_final, dim3(1), dim3(64 * 7), 0, stream, nblk, partial, fin);
   PMX_HIP(hipGetLastError());
   PMX_HIP(hipMemcpyAsync(out7, fin, sizeof(double) * 7, hipMemcpyDeviceToHost, stream));
   if (nw > 0 && wells_out)
     PMX_HIP(hipMemcpyAsync(wells_out, r.p + 2 * (size_t)nc, sizeof(double) * 3 * nw, hipMemcpyDeviceToHost, stream));
   PMX_HIP(hipStreamSynchronize(stream));
+}
+
+// Allocation-free convergence reductions.  The partials live in `scratch`,
+// one copy per GPU for the life of the process: kConvBlocks rows of 7
+// doubles plus the 7 results (28,736 bytes).  Its device address is looked
+// up once per GPU, with that GPU current, and only that address is handed
+// to kernels and copies.  Calls that overlap on one GPU share the array and
+// race.
+constexpr size_t kConvScratchDoubles = (size_t)kConvBlocks * 7 + 8;
+static __device__ __align__(256) double scratch[kConvScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static double* scratch_address(int device) {
+  PMX_CHECK(device >= 0 && device < kMaxDevices, "device id out of range for the static scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* ptr = nullptr;
+    PMX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    PMX_CHECK(ptr != nullptr, "no device address for the static scratch array");
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+void OWModelDev::convergence_noalloc(int device, const double* p, double* out7, double* wells_out) {
+  PMX_CHECK(nc > 0 && r.n >= (size_t)2 * nc, "assemble before the convergence reductions");
+  // one partial row per 2,048 cells, at most kConvBlocks rows
+  const int nblk = (int)std::min<size_t>(kConvBlocks, ((size_t)nc + 2047) / 2048);
+  PMX_CHECK((size_t)nblk * 7 + 7 <= kConvScratchDoubles,
+            "convergence reductions need more partial rows than the static scratch holds");
+  reduce_convergence(p, out7, wells_out, scratch_address(device), nblk);
 }
 
 int OWModelDev::solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean) {
