@@ -284,6 +284,34 @@ int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const do
                      const double* wdp, const pmx_solver_options* o, double* p, double* flux, double* face_pressure,
                      double* well_p, double* well_flux, pmx_solve_info* info);
 
+/* The stages of pmx_incomp_solve, for the reference's MatrixOutput and
+ * LinSolve options (incomp_tpfa.py:55-63, 172-184):
+ * pmx_incomp_assemble builds the system A x = rhs of incomp_tpfa
+ * (:120-171, the null-space pin included) from the same inputs as
+ * pmx_incomp_solve, plus periodic-boundary pressure jumps (bcp: nbcp faces
+ * and values added across those faces, :113-118);
+ * pmx_incomp_nnz / pmx_incomp_export copy it to host arrays: the cell
+ * matrix in CSR (rowptr[nc+1], col[nnz], val[nnz]), per perforation the
+ * cell-row/well-column and well-row/cell-column entries Bp[nperf],
+ * Cp[nperf], the well diagonal D[nw] and rhs[nc+nw] (any pointer may be
+ * NULL); pmx_incomp_solve_assembled solves it on the device into x
+ * (device, nc + nw: cell pressures, then well pressures);
+ * pmx_incomp_reconstruct computes pressures, face pressures, fluxes and
+ * the wells' pressures and connection fluxes from any solution x (device)
+ * of the last assembled system (:186-210). */
+int pmx_incomp_assemble(pmx_incomp* h, const double* T, const double* mob, const double* omega, const double* gravity,
+                        int32_t nbc, const int32_t* bc_face, const int32_t* bc_type, const double* bc_value,
+                        int32_t nsrc, const int32_t* src_cell, const double* src_rate, int32_t nbcp,
+                        const int32_t* bcp_face, const double* bcp_value, int32_t nw, const int32_t* perf_ptr,
+                        const int32_t* perf_cells, const double* WI, const int32_t* wtype, const double* wval,
+                        const double* wdp);
+int pmx_incomp_nnz(pmx_incomp* h, int32_t* nnz);
+int pmx_incomp_export(pmx_incomp* h, int32_t* rowptr, int32_t* col, double* val, double* Bp, double* Cp, double* D,
+                      double* rhs);
+int pmx_incomp_solve_assembled(pmx_incomp* h, const pmx_solver_options* o, double* x, pmx_solve_info* info);
+int pmx_incomp_reconstruct(pmx_incomp* h, const double* x, double* p, double* flux, double* face_pressure,
+                           double* well_p, double* well_flux);
+
 /* ---- a caller's linear system -----------------------------------------
  * Replaces the reference's linear-solver seam: CPRSolver.solve(jac, rhs,
  * model, state, dt) (pymrst/modules/ad_core/solvers/linear_solvers.py:426-545)
@@ -311,6 +339,16 @@ int pmx_sys_matvec(pmx_sys* h, const double* x, double* y);
  * 1 not converged, <0 error. */
 int pmx_sys_solve(pmx_sys* h, const pmx_solver_options* o, const double* w, const double* rhs, double* x,
                   pmx_solve_info* info);
+
+/* The CPR elliptic-solver injection of the reference
+ * (linear_solvers.py:124-131, 253-256: elliptic_solver.solve_linear_system
+ * (A_p, r)): fn(user, n, b, x) solves the pressure system A_p x = b for
+ * the n cells (host arrays; A_p = W J P with the quasi-IMPES weights of
+ * pmx_sys_solve) once per preconditioner application, in place of the
+ * device AMG V-cycle, and returns 0 (nonzero fails the solve).  fn NULL
+ * returns to the AMG.  Applications then run without graph replay. */
+typedef int (*pmx_pressure_solver)(void* user, int32_t n, const double* b, double* x);
+int pmx_sys_set_pressure_solver(pmx_sys* h, pmx_pressure_solver fn, void* user);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,13 @@ SIGNATURES = {
     "pmx_incomp_solve": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _I, _P, _P, _P, _P, _P, _P,
                                         ctypes.POINTER(SolverOptions), _P, _P, _P, _P, _P,
                                         ctypes.POINTER(SolveInfo)]),
+    "pmx_incomp_assemble": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, _P, _P, _P, _I, _P, _P, _I, _P, _P, _I, _P, _P,
+                                           _P, _P, _P, _P]),
+    "pmx_incomp_nnz": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
+    "pmx_sys_set_pressure_solver": (ctypes.c_int, [_P, _P, _P]),
+    "pmx_incomp_export": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "pmx_incomp_solve_assembled": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, ctypes.POINTER(SolveInfo)]),
+    "pmx_incomp_reconstruct": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -369,6 +369,41 @@ int pmx_compute_trans(pmx_incomp* h, const double* face_normals, int32_t nk, con
   })
 }
 
+static IncompBC incomp_bc(int32_t nbc, const int32_t* bc_face, const int32_t* bc_type, const double* bc_value,
+                           int32_t nsrc, const int32_t* src_cell, const double* src_rate) {
+  IncompBC bc;
+  if (nbc > 0) {
+    PMX_CHECK(bc_face && bc_type && bc_value, "null boundary-condition argument");
+    bc.face.assign(bc_face, bc_face + nbc);
+    bc.type.assign(bc_type, bc_type + nbc);
+    bc.value.assign(bc_value, bc_value + nbc);
+  }
+  if (nsrc > 0) {
+    PMX_CHECK(src_cell && src_rate, "null source argument");
+    bc.src_cell.assign(src_cell, src_cell + nsrc);
+    bc.src_rate.assign(src_rate, src_rate + nsrc);
+  }
+  return bc;
+}
+
+static IncompWells incomp_wells(const pmx_incomp* h, int32_t nw, const int32_t* perf_ptr, const int32_t* perf_cells,
+                                const double* WI, const int32_t* wtype, const double* wval, const double* wdp) {
+  IncompWells W;
+  if (nw > 0) {
+    PMX_CHECK(perf_ptr && perf_cells && WI && wtype && wval && wdp, "null well argument");
+    const int np = perf_ptr[nw];
+    W.perf_ptr.assign(perf_ptr, perf_ptr + nw + 1);
+    W.cells.assign(perf_cells, perf_cells + np);
+    W.WI.assign(WI, WI + np);
+    W.dp.assign(wdp, wdp + np);
+    W.type.assign(wtype, wtype + nw);
+    W.val.assign(wval, wval + nw);
+    for (int c : W.cells) PMX_CHECK(c >= 0 && c < h->m->nc, "well connection cell out of range");
+    for (int t : W.type) PMX_CHECK(t == 0 || t == 1, "unsupported well control");
+  }
+  return W;
+}
+
 int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const double* omega, const double* gravity,
                      int32_t nbc, const int32_t* bc_face, const int32_t* bc_type, const double* bc_value, int32_t nsrc,
                      const int32_t* src_cell, const double* src_rate, int32_t nw, const int32_t* perf_ptr,
@@ -377,42 +412,14 @@ int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const do
                      double* well_p, double* well_flux, pmx_solve_info* info) {
   try {
     PMX_CHECK(h && T && mob && omega && p && flux && face_pressure, "null argument");
+    PMX_CHECK(nw == 0 || (well_p && well_flux), "null well output");
     PMX_HIP(hipSetDevice(h->device));
-    IncompBC bc;
-    if (nbc > 0) {
-      bc.face.assign(bc_face, bc_face + nbc);
-      bc.type.assign(bc_type, bc_type + nbc);
-      bc.value.assign(bc_value, bc_value + nbc);
-    }
-    if (nsrc > 0) {
-      bc.src_cell.assign(src_cell, src_cell + nsrc);
-      bc.src_rate.assign(src_rate, src_rate + nsrc);
-    }
-    IncompWells W;
-    if (nw > 0) {
-      PMX_CHECK(perf_ptr && perf_cells && WI && wtype && wval && wdp && well_p && well_flux, "null well argument");
-      const int np = perf_ptr[nw];
-      W.perf_ptr.assign(perf_ptr, perf_ptr + nw + 1);
-      W.cells.assign(perf_cells, perf_cells + np);
-      W.WI.assign(WI, WI + np);
-      W.dp.assign(wdp, wdp + np);
-      W.type.assign(wtype, wtype + nw);
-      W.val.assign(wval, wval + nw);
-      for (int c : W.cells) PMX_CHECK(c >= 0 && c < h->m->nc, "well connection cell out of range");
-      for (int t : W.type) PMX_CHECK(t == 0 || t == 1, "unsupported well control");
-    }
+    const IncompBC bc = incomp_bc(nbc, bc_face, bc_type, bc_value, nsrc, src_cell, src_rate);
+    const IncompWells W = incomp_wells(h, nw, perf_ptr, perf_cells, WI, wtype, wval, wdp);
     SolveInfo si;
     const int st = h->m->solve(T, mob, omega, gravity, bc, W, to_opts(o), p, flux, face_pressure, well_p,
                                well_flux, si);
-    if (info) {
-      info->iterations = si.iterations;
-      info->restarts = si.restarts;
-      info->rel_residual = si.rel_residual;
-      info->setup_ms = si.setup_ms;
-      info->solve_ms = si.solve_ms;
-      info->levels = si.levels;
-      info->coarse_n = si.coarse_n;
-    }
+    fill_info(info, si);
     if (st < 0) {
       g_err = "pressure solve produced a non-finite residual";
       return -2;
@@ -422,6 +429,69 @@ int pmx_incomp_solve(pmx_incomp* h, const double* T, const double* mob, const do
     g_err = e.what();
     return -1;
   }
+}
+
+int pmx_incomp_assemble(pmx_incomp* h, const double* T, const double* mob, const double* omega,
+                        const double* gravity, int32_t nbc, const int32_t* bc_face, const int32_t* bc_type,
+                        const double* bc_value, int32_t nsrc, const int32_t* src_cell, const double* src_rate,
+                        int32_t nbcp, const int32_t* bcp_face, const double* bcp_value, int32_t nw,
+                        const int32_t* perf_ptr, const int32_t* perf_cells, const double* WI, const int32_t* wtype,
+                        const double* wval, const double* wdp) {
+  PMX_TRY({
+    PMX_CHECK(h && T && mob && omega, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    IncompBC bc = incomp_bc(nbc, bc_face, bc_type, bc_value, nsrc, src_cell, src_rate);
+    if (nbcp > 0) {
+      PMX_CHECK(bcp_face && bcp_value, "null periodic boundary-condition argument");
+      bc.jump_face.assign(bcp_face, bcp_face + nbcp);
+      bc.jump_value.assign(bcp_value, bcp_value + nbcp);
+    }
+    h->m->assemble(T, mob, omega, gravity, bc, incomp_wells(h, nw, perf_ptr, perf_cells, WI, wtype, wval, wdp));
+  })
+}
+
+int pmx_incomp_nnz(pmx_incomp* h, int32_t* nnz) {
+  PMX_TRY({
+    PMX_CHECK(h && nnz, "null argument");
+    *nnz = h->m->S.nnzb;
+  })
+}
+
+int pmx_incomp_export(pmx_incomp* h, int32_t* rowptr, int32_t* col, double* val, double* Bp, double* Cp, double* D,
+                      double* rhs) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    h->m->export_system(rowptr, col, val, Bp, Cp, D, rhs);
+  })
+}
+
+int pmx_incomp_solve_assembled(pmx_incomp* h, const pmx_solver_options* o, double* x, pmx_solve_info* info) {
+  try {
+    PMX_CHECK(h && x, "null argument");
+    PMX_HIP(hipSetDevice(h->device));
+    SolveInfo si;
+    const int st = h->m->solve_assembled(to_opts(o), x, si);
+    fill_info(info, si);
+    if (st < 0) {
+      g_err = "pressure solve produced a non-finite residual";
+      return -2;
+    }
+    return st;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+int pmx_incomp_reconstruct(pmx_incomp* h, const double* x, double* p, double* flux, double* face_pressure,
+                           double* well_p, double* well_flux) {
+  PMX_TRY({
+    PMX_CHECK(h && x && p && flux && face_pressure, "null argument");
+    PMX_CHECK(h->m->wells.type.empty() || (well_p && well_flux), "null well output");
+    PMX_HIP(hipSetDevice(h->device));
+    h->m->reconstruct(x, p, flux, face_pressure, well_p, well_flux);
+  })
 }
 
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z) {
@@ -801,6 +871,13 @@ int pmx_sys_matvec(pmx_sys* h, const double* x, double* y) {
     return -1;
   }
   return 0;
+}
+
+int pmx_sys_set_pressure_solver(pmx_sys* h, pmx_pressure_solver fn, void* user) {
+  PMX_TRY({
+    PMX_CHECK(h != nullptr, "null argument");
+    cpr_set_pressure_solver(h->cpr, fn, user);
+  })
 }
 
 int pmx_sys_solve(pmx_sys* h, const pmx_solver_options* opts, const double* w, const double* rhs, double* x,

@@ -9,6 +9,8 @@ struct IncompBC {
   std::vector<double> value;
   std::vector<int> src_cell;
   std::vector<double> src_rate;
+  std::vector<int> jump_face;      // periodic boundary: pressure jump added across these faces
+  std::vector<double> jump_value;
 };
 
 struct IncompWells {
@@ -28,6 +30,14 @@ class IncompDev {
   int solve(const double* T, const double* mob, const double* omega, const double* g, const IncompBC& bc,
             const IncompWells& W, const LinearSolverOptions& o, double* p, double* flux, double* fpress,
             double* well_p, double* well_flux, SolveInfo& info);
+  // the stages of solve: assembly, the device solve into x (nc + nw), and
+  // the reconstruction from any solution x; export_system copies the
+  // assembled system to the host
+  void assemble(const double* T, const double* mob, const double* omega, const double* g, const IncompBC& bc,
+                const IncompWells& W);
+  int solve_assembled(const LinearSolverOptions& o, double* x, SolveInfo& info);
+  void reconstruct(const double* x, double* p, double* flux, double* fpress, double* well_p, double* well_flux);
+  void export_system(int* rowptr, int* col, double* val, double* Bp, double* Cp, double* D, double* rhs);
 
   hipStream_t stream;
   int nc = 0, nf = 0, nhf = 0;
@@ -36,7 +46,13 @@ class IncompDev {
   DArr<int> face_pos, hf_face, hf_cell, face_nb, face_hf;
   DArr<double> cc, fc, thf, ghf, ft, fg, fg_all, dval, hh, gg, r, xw, wi_d, radd_d, w1;
   DArr<unsigned char> dF;
+  DArr<int> jump_face;
+  DArr<double> jump_val;
   CprSolver* cpr = nullptr;
+  // the last assembly's wells and their host-side coefficients
+  bool assembled = false;
+  IncompWells wells;
+  std::vector<double> wi_h, Bp_h, Cp_h, D_h;
 
  private:
   double* x_work();

@@ -180,6 +180,15 @@ __global__ void k_incomp_reconstruct(int nf, const int* __restrict__ face_hf, co
   }
 }
 
+// Periodic-boundary pressure jumps (incomp_tpfa.py:113-118): added to the
+// gravity term of their faces, so they enter the right-hand side and the
+// interior fluxes.
+__global__ void k_add_face_jump(int n, const int* __restrict__ face, const double* __restrict__ jump,
+                                double* __restrict__ fg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) fg[face[i]] += jump[i];
+}
+
 __global__ void k_scale_entry(double* __restrict__ val, int k, double a) {
   if (threadIdx.x == 0 && blockIdx.x == 0) val[k] *= a;
 }
@@ -248,9 +257,8 @@ void IncompDev::compute_trans(const double* fn, int nk, const double* perm, doub
   PMX_HIP(hipGetLastError());
 }
 
-int IncompDev::solve(const double* T, const double* mob, const double* omega, const double* g, const IncompBC& bc,
-                     const IncompWells& W, const LinearSolverOptions& o, double* p, double* flux, double* fpress,
-                     double* well_p, double* well_flux, SolveInfo& info) {
+void IncompDev::assemble(const double* T, const double* mob, const double* omega, const double* g,
+                         const IncompBC& bc, const IncompWells& W) {
   // boundary conditions and sources (compute_pressure_rhs semantics)
   std::vector<unsigned char> dF_h(nf, 0);
   std::vector<double> dval_h(nf, 0.0), hh_h(nf, 0.0), gg_h(nc, 0.0);
@@ -282,6 +290,14 @@ int IncompDev::solve(const double* T, const double* mob, const double* omega, co
   hipLaunchKernelGGL(k_incomp_face, dim3(grid_for(nf)), dim3(256), 0, stream, nf, face_hf.p, face_nb.p, hf_cell.p,
                      thf.p, ghf.p, dF.p, ft.p, fg.p, fg_all.p);
   PMX_HIP(hipGetLastError());
+  if (!bc.jump_face.empty()) {
+    for (int f : bc.jump_face) PMX_CHECK(f >= 0 && f < nf, "periodic boundary condition on a face out of range");
+    jump_face.upload(bc.jump_face, stream);
+    jump_val.upload(bc.jump_value, stream);
+    hipLaunchKernelGGL(k_add_face_jump, dim3(grid_for(bc.jump_face.size())), dim3(256), 0, stream,
+                       (int)bc.jump_face.size(), jump_face.p, jump_val.p, fg.p);
+    PMX_HIP(hipGetLastError());
+  }
   // wells: host part (per connection wi = WI mob, right-hand sides)
   const int nw = (int)W.type.size();
   if (nw != S.nw || (nw && W.perf_ptr[nw] != S.nperf)) {
@@ -293,7 +309,12 @@ int IncompDev::solve(const double* T, const double* mob, const double* omega, co
   hipLaunchKernelGGL(k_incomp_rows, dim3(grid_for(nc)), dim3(256), 0, stream, nc, face_pos.p, hf_face.p, face_nb.p,
                      thf.p, ft.p, fg.p, dF.p, dval.p, hh.p, gg.p, S.rowptr.p, S.col.p, S.diag.p, S.val.p, r.p);
   PMX_HIP(hipGetLastError());
-  std::vector<double> wi;
+  std::vector<double>& wi = wi_h;
+  wells = W;
+  wi.clear();
+  Bp_h.clear();
+  Cp_h.clear();
+  D_h.clear();
   if (nw > 0) {
     const int np = S.nperf;
     std::vector<double> mob_perf(np);
@@ -333,6 +354,9 @@ int IncompDev::solve(const double* T, const double* mob, const double* omega, co
     PMX_HIP(hipMemcpyAsync(S.Bp.p, Bp.data(), sizeof(double) * np, hipMemcpyHostToDevice, stream));
     PMX_HIP(hipMemcpyAsync(S.Cp.p, Cp.data(), sizeof(double) * np, hipMemcpyHostToDevice, stream));
     PMX_HIP(hipMemcpyAsync(S.Dp.p, D.data(), sizeof(double) * nw, hipMemcpyHostToDevice, stream));
+    Bp_h = Bp;
+    Cp_h = Cp;
+    D_h = D;
     PMX_HIP(hipMemcpyAsync(r.p + nc, rw.data(), sizeof(double) * nw, hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(k_incomp_perfs, dim3(grid_for(S.npc)), dim3(256), 0, stream, S.npc, S.pc_cell.p, S.pc_ptr.p,
                        S.pc_perf.p, S.diag.p, wi_d.p, radd_d.p, S.val.p, r.p);
@@ -355,7 +379,13 @@ int IncompDev::solve(const double* T, const double* mob, const double* omega, co
     hipLaunchKernelGGL(k_scale_entry, dim3(1), dim3(64), 0, stream, S.val.p, k, 2.0);
     PMX_HIP(hipGetLastError());
   }
-  // solve: AMG on the pressure matrix inside FGMRES
+  PMX_HIP(hipStreamSynchronize(stream));
+  assembled = true;
+}
+
+int IncompDev::solve_assembled(const LinearSolverOptions& o, double* x, SolveInfo& info) {
+  PMX_CHECK(assembled, "assemble before solving");
+  // AMG on the pressure matrix inside FGMRES
   if (!cpr_has_hierarchy(cpr)) {
     std::vector<double> vals(S.nnzb);
     PMX_HIP(hipMemcpyAsync(vals.data(), S.val.p, sizeof(double) * S.nnzb, hipMemcpyDeviceToHost, stream));
@@ -367,24 +397,56 @@ int IncompDev::solve(const double* T, const double* mob, const double* omega, co
     cpr_build_hierarchy(cpr, S, wts, o, stream);
   }
   cpr_setup(cpr, S, w1.p, o, stream);
-  const int st = cpr_solve(cpr, S, r.p, x_work(), o, info, stream);
-  // unknowns: cell pressures then bhp per well
-  PMX_HIP(hipMemcpyAsync(p, xw.p, sizeof(double) * nc, hipMemcpyDeviceToDevice, stream));
+  return cpr_solve(cpr, S, r.p, x, o, info, stream);
+}
+
+// Pressures, face pressures, fluxes and well quantities from a solution x
+// (device: cell pressures, then the pressure of each well) of the system
+// last assembled (incomp_tpfa.py:365-389).
+void IncompDev::reconstruct(const double* x, double* p, double* flux, double* fpress, double* well_p,
+                            double* well_flux) {
+  PMX_CHECK(assembled, "assemble before reconstructing");
+  const IncompWells& W = wells;
+  const int nw = (int)W.type.size();
+  PMX_HIP(hipMemcpyAsync(p, x, sizeof(double) * nc, hipMemcpyDeviceToDevice, stream));
   hipLaunchKernelGGL(k_incomp_reconstruct, dim3(grid_for(nf)), dim3(256), 0, stream, nf, face_hf.p, face_nb.p,
-                     hf_cell.p, thf.p, ghf.p, ft.p, fg.p, fg_all.p, dF.p, dval.p, hh.p, xw.p, fpress, flux);
+                     hf_cell.p, thf.p, ghf.p, ft.p, fg.p, fg_all.p, dF.p, dval.p, hh.p, x, fpress, flux);
   PMX_HIP(hipGetLastError());
   if (nw > 0) {
     std::vector<double> x_h(S.n());
-    PMX_HIP(hipMemcpyAsync(x_h.data(), xw.p, sizeof(double) * S.n(), hipMemcpyDeviceToHost, stream));
+    PMX_HIP(hipMemcpyAsync(x_h.data(), x, sizeof(double) * S.n(), hipMemcpyDeviceToHost, stream));
     PMX_HIP(hipStreamSynchronize(stream));
     for (int w = 0; w < nw; ++w) {
       well_p[w] = x_h[nc + w];
       for (int j = W.perf_ptr[w]; j < W.perf_ptr[w + 1]; ++j)
-        well_flux[j] = wi[j] * (x_h[nc + w] + W.dp[j] - x_h[W.cells[j]]);
+        well_flux[j] = wi_h[j] * (x_h[nc + w] + W.dp[j] - x_h[W.cells[j]]);
     }
   }
   PMX_HIP(hipStreamSynchronize(stream));
+}
+
+int IncompDev::solve(const double* T, const double* mob, const double* omega, const double* g, const IncompBC& bc,
+                     const IncompWells& W, const LinearSolverOptions& o, double* p, double* flux, double* fpress,
+                     double* well_p, double* well_flux, SolveInfo& info) {
+  assemble(T, mob, omega, g, bc, W);
+  const int st = solve_assembled(o, x_work(), info);
+  reconstruct(xw.p, p, flux, fpress, well_p, well_flux);
   return st;
+}
+
+// The assembled system on the host: the cell matrix in CSR (rowptr[nc+1],
+// col, val), per perforation the cell-row / well-column and well-row /
+// cell-column entries (Bp, Cp), the well diagonal D[nw], and rhs[nc+nw].
+void IncompDev::export_system(int* rowptr, int* col, double* val, double* Bp, double* Cp, double* D, double* rhs) {
+  PMX_CHECK(assembled, "assemble before exporting the system");
+  if (rowptr) std::copy(host_pat.rowptr.begin(), host_pat.rowptr.end(), rowptr);
+  if (col) std::copy(host_pat.col.begin(), host_pat.col.end(), col);
+  if (val) PMX_HIP(hipMemcpyAsync(val, S.val.p, sizeof(double) * S.nnzb, hipMemcpyDeviceToHost, stream));
+  if (rhs) PMX_HIP(hipMemcpyAsync(rhs, r.p, sizeof(double) * S.n(), hipMemcpyDeviceToHost, stream));
+  PMX_HIP(hipStreamSynchronize(stream));
+  if (Bp) std::copy(Bp_h.begin(), Bp_h.end(), Bp);
+  if (Cp) std::copy(Cp_h.begin(), Cp_h.end(), Cp);
+  if (D) std::copy(D_h.begin(), D_h.end(), D);
 }
 
 double* IncompDev::x_work() {

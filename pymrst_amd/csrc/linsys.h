@@ -103,6 +103,11 @@ bool cpr_has_hierarchy(const CprSolver*);
 // Lines for the local stage (line_ptr[nlines+1], cells); without them lines
 // are detected from the strength weights.
 void cpr_set_lines(CprSolver*, int nlines, const int* ptr, const int* cells);
+// A host solver for the pressure stage, x = A_p^{-1} b (n cells, host
+// arrays), called once per preconditioner application in place of the AMG
+// V-cycle; fn == nullptr returns to the AMG.  Returns 0 on success.
+typedef int (*PressureSolveFn)(void* user, int n, const double* b, double* x);
+void cpr_set_pressure_solver(CprSolver*, PressureSolveFn fn, void* user);
 // Independent segments (ensemble members): cells [cell_ptr[s], cell_ptr[s+1])
 // and wells [well_ptr[s], well_ptr[s+1]) form segment s; FGMRES then keeps
 // one Krylov recurrence per segment.

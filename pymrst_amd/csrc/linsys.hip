@@ -1630,7 +1630,17 @@ class CprSolver {
   // repeated pressure cycles: original right-hand side and accumulated solution
   DArr<double> pb64, pacc64;
   DArr<float> pb32, pacc32;
+  // a caller's pressure-stage solver (host callback) in place of the AMG
+  PressureSolveFn pfn = nullptr;
+  void* puser = nullptr;
+  std::vector<double> pb_host, px_host;
 };
+
+void cpr_set_pressure_solver(CprSolver* C, PressureSolveFn fn, void* user) {
+  C->pfn = fn;
+  C->puser = user;
+  C->graph_ok = false;
+}
 
 void cpr_profile(CprSolver* C, bool on) {
   C->spmv_t.on = on;
@@ -2053,6 +2063,22 @@ static void apply_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
   // stage 1: pressure
   hipLaunchKernelGGL((k_weight_restrict<B, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, wdev, C->io, V::b(f).p);
   launch_check();
+  if (C->pfn != nullptr) {
+    // the caller's pressure solver on the host: b = W r out, x back in
+    const size_t n = f.n;
+    std::vector<T> tb(n);
+    PMX_HIP(hipMemcpyAsync(tb.data(), V::b(f).p, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+    PMX_HIP(hipStreamSynchronize(s));
+    C->pb_host.assign(tb.begin(), tb.end());
+    C->px_host.assign(n, 0.0);
+    const int rc = C->pfn(C->puser, (int)n, C->pb_host.data(), C->px_host.data());
+    PMX_CHECK(rc == 0, "the pressure-stage solver reported a failure");
+    for (size_t i = 0; i < n; ++i) tb[i] = (T)C->px_host[i];
+    PMX_HIP(hipMemcpyAsync(V::x(f).p, tb.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
+    PMX_HIP(hipStreamSynchronize(s));
+    stage2<B, M, X, T>(C, S, o, s);
+    return;
+  }
   vcycle<T>(C, 0, o, s);
   if (o.amg_cycles > 1) {
     // stationary iteration with the V-cycle: x <- x + V(b - A_p x)
@@ -2209,7 +2235,7 @@ void cpr_apply(CprSolver* C, const BlockSystem& S, const double* r, double* z, c
   if (C->io == nullptr) PMX_HIP(hipMalloc(&C->io, sizeof(PrecIO)));
   hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, s, C->io, r, z);
   launch_check();
-  if (!o.use_graphs) {
+  if (!o.use_graphs || C->pfn != nullptr) {   // a host pressure solver cannot be replayed from a graph
     apply_sequence(C, S, o, s);
     return;
   }

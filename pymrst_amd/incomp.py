@@ -1,8 +1,9 @@
 """Incompressible TPFA pressure solve on the MI355X (incompTPFA), fed by a
 device computeTrans.
 
-Reference: incomp_tpfa_ad, pymrst/modules/incomp/incomp_tpfa.py:252-389
-(the variant MRST's validation replays use), with compute_pressure_rhs
+Reference: incomp_tpfa (pymrst/modules/incomp/incomp_tpfa.py:55-211, the
+full option surface) and incomp_tpfa_ad (:252-389, the variant MRST's
+validation replays use), with compute_pressure_rhs
 semantics (:392-435), incomp_well_pressure_drop
 (pymrst/params/wells_and_bc/incomp_well_pressure_drop.py:9-26) and
 compute_trans (pymrst/solvers/compute_trans.py:11-29).  The transmissibility,
@@ -59,6 +60,8 @@ class State:
     s: np.ndarray
     facePressure: np.ndarray = None
     wellSol: list = field(default_factory=list)
+    A: object = None        # the system and its right-hand side, with MatrixOutput or reduce
+    rhs: np.ndarray = None
 
 
 def init_res_sol(g: Grid, p0, s0=1.0) -> State:
@@ -160,7 +163,14 @@ class IncompTPFA:
                                               _lib.ptr(perm), _lib.ptr(T)), "pmx_compute_trans")
         return T
 
-    def solve(self, state: State, T, fluid: IncompFluid, bc=None, src=None, wells=None, gravity=None) -> State:
+    def solve(self, state: State, T, fluid: IncompFluid, bc=None, src=None, wells=None, gravity=None, *,
+              bcp=None, lin_solve=None, matrix_output=False, reduce=False, use_trans=False,
+              condition_number=False, well_gravity=None) -> State:
+        """Assemble on the device, solve (on the device, or by `lin_solve(A,
+        b)` on the exported system), reconstruct on the device.
+        `well_gravity`: the gravity magnitude of the wells' hydrostatic
+        drops (default: that of `gravity`)."""
+        import scipy.sparse as sp
         g = self.g
         nc, nf = g.num_cells, g.num_faces
         wells = list(wells or [])
@@ -169,7 +179,11 @@ class IncompTPFA:
         totmob = mob.sum(axis=1)
         omega = (mob * fluid.rho).sum(axis=1) / totmob
         dev = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=self.device)
-        T_d = T if isinstance(T, torch.Tensor) else dev(T)
+        if use_trans:
+            T_d = dev(self._faces_to_half_faces(np.asarray(T.cpu() if isinstance(T, torch.Tensor) else T, float),
+                                                totmob))
+        else:
+            T_d = T if isinstance(T, torch.Tensor) else dev(T)
         mob_d, om_d = dev(totmob), dev(omega)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
@@ -184,9 +198,15 @@ class IncompTPFA:
         else:
             s_cell = s_rate = None
             nsrc = 0
+        if bcp is not None and np.size(bcp.face):
+            j_face, j_val, njump = i32(bcp.face), f64(np.broadcast_to(bcp.value, np.shape(bcp.face))), np.size(bcp.face)
+        else:
+            j_face = j_val = None
+            njump = 0
         nw = len(wells)
         norm_g = float(np.linalg.norm(gvec))
-        cdp = [well_pressure_drop(w, mob, fluid.rho, norm_g) for w in wells]
+        wg = norm_g if well_gravity is None else float(well_gravity)
+        cdp = [well_pressure_drop(w, mob, fluid.rho, wg) for w in wells]
         if nw:
             perf_ptr = i32(np.concatenate([[0], np.cumsum([w.cells.size for w in wells])]))
             perf_cells = i32(np.concatenate([w.cells for w in wells]))
@@ -198,31 +218,103 @@ class IncompTPFA:
             well_flux = np.zeros(perf_cells.size)
         else:
             perf_ptr = perf_cells = WI = wtype = wval = wdp = well_p = well_flux = None
+        P = _lib.ptr
+        _lib.check(self.lib.pmx_incomp_assemble(self.h, P(T_d), P(mob_d), P(om_d),
+                                                P(f64(gvec)) if norm_g > 0 else P(None), nbc, P(bc_face),
+                                                P(bc_type), P(bc_val), nsrc, P(s_cell), P(s_rate), njump,
+                                                P(j_face), P(j_val), nw, P(perf_ptr), P(perf_cells), P(WI),
+                                                P(wtype), P(wval), P(wdp)), "pmx_incomp_assemble")
+        A = rhs = None
+        if matrix_output or lin_solve is not None or reduce or condition_number:
+            A, rhs = self._export(nw, perf_ptr, perf_cells)
+            if condition_number:
+                print(f"Condition number is: {np.linalg.cond(A.toarray())}")
+        x = torch.empty(nc + nw, dtype=torch.float64, device=self.device)
+        A_out, rhs_out = A, rhs
+        if reduce and nw:
+            # Schur complement of the well unknowns (incomp_tpfa.py:173-182)
+            C, Dw = A[nc:, :nc], A[nc:, nc:].diagonal()
+            Dinv = sp.diags_array(1.0 / Dw)
+            A_out = (A[:nc, :nc] - C.T @ Dinv @ C).tocsr()
+            rhs_out = rhs[:nc] - C.T @ (Dinv @ rhs[nc:])
+        if lin_solve is not None:
+            xs = np.asarray(lin_solve(A_out, rhs_out), dtype=float).ravel()
+            if reduce and nw:
+                xs = np.concatenate([xs, (rhs[nc:] - A[nc:, :nc] @ xs) / A[nc:, nc:].diagonal()])
+            x.copy_(torch.as_tensor(xs, device=self.device))
+            self.last_info = None
+        else:
+            o = _lib.solver_defaults()
+            o.rtol, o.maxiter, o.restart = self.rtol, self.maxiter, 100
+            info = _lib.SolveInfo()
+            rc = self.lib.pmx_incomp_solve_assembled(self.h, ctypes.byref(o), P(x), ctypes.byref(info))
+            _lib.check(rc, "pmx_incomp_solve_assembled")
+            if rc != 0:
+                raise RuntimeError(f"incompTPFA pressure solve did not converge ({info.iterations} its, "
+                                   f"rel residual {info.rel_residual:.1e})")
+            self.last_info = info
         p = torch.empty(nc, dtype=torch.float64, device=self.device)
         flux = torch.empty(nf, dtype=torch.float64, device=self.device)
         fpress = torch.empty(nf, dtype=torch.float64, device=self.device)
-        o = _lib.solver_defaults()
-        o.rtol, o.maxiter, o.restart = self.rtol, self.maxiter, 100
-        info = _lib.SolveInfo()
-        P = _lib.ptr
-        rc = self.lib.pmx_incomp_solve(self.h, P(T_d), P(mob_d), P(om_d), P(f64(gvec)) if norm_g > 0 else P(None),
-                                       nbc, P(bc_face), P(bc_type), P(bc_val), nsrc, P(s_cell), P(s_rate), nw,
-                                       P(perf_ptr), P(perf_cells), P(WI), P(wtype), P(wval), P(wdp),
-                                       ctypes.byref(o), P(p), P(flux), P(fpress), P(well_p), P(well_flux),
-                                       ctypes.byref(info))
-        _lib.check(rc, "pmx_incomp_solve")
-        if rc != 0:
-            raise RuntimeError(f"incompTPFA pressure solve did not converge ({info.iterations} its, "
-                               f"rel residual {info.rel_residual:.1e})")
-        self.last_info = info
+        _lib.check(self.lib.pmx_incomp_reconstruct(self.h, P(x), P(p), P(flux), P(fpress), P(well_p), P(well_flux)),
+                   "pmx_incomp_reconstruct")
         out = State(pressure=p.cpu().numpy(), flux=flux.cpu().numpy(), s=state.s,
                     facePressure=fpress.cpu().numpy())
+        if matrix_output or reduce:
+            out.A, out.rhs = A_out, rhs_out
         off = 0
         for k, w in enumerate(wells):
             n = w.cells.size
             out.wellSol.append(WellSol(flux=well_flux[off:off + n].copy(), pressure=float(well_p[k]), cdp=cdp[k]))
             off += n
         return out
+
+    def _export(self, nw, perf_ptr, perf_cells):
+        """The last assembled system as a SciPy CSR matrix over [cells, wells]
+        and its right-hand side."""
+        import scipy.sparse as sp
+        nc = self.g.num_cells
+        nnz = ctypes.c_int32()
+        _lib.check(self.lib.pmx_incomp_nnz(self.h, ctypes.byref(nnz)), "pmx_incomp_nnz")
+        rowptr = np.zeros(nc + 1, np.int32)
+        col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        npf = 0 if not nw else int(perf_ptr[-1])
+        Bp, Cp, D = np.zeros(max(npf, 1)), np.zeros(max(npf, 1)), np.zeros(max(nw, 1))
+        rhs = np.zeros(nc + nw)
+        P = _lib.ptr
+        _lib.check(self.lib.pmx_incomp_export(self.h, P(rowptr), P(col), P(val), P(Bp), P(Cp), P(D), P(rhs)),
+                   "pmx_incomp_export")
+        rows = np.repeat(np.arange(nc), np.diff(rowptr))
+        if nw:
+            wrow = np.repeat(np.arange(nw), np.diff(perf_ptr)) + nc
+            r = np.concatenate([rows, perf_cells, wrow, nc + np.arange(nw)])
+            c = np.concatenate([col, wrow, perf_cells, nc + np.arange(nw)])
+            v = np.concatenate([val, Bp[:npf], Cp[:npf], D[:nw]])
+        else:
+            r, c, v = rows, col, val
+        A = sp.csr_array((v, (r, c)), shape=(nc + nw, nc + nw))
+        A.sum_duplicates()
+        A.eliminate_zeros()
+        return A, rhs
+
+    def _faces_to_half_faces(self, T_face, totmob):
+        """One transmissibility per interface (use_trans) as the one-sided
+        values the device assembly takes: the reference's face value
+        T_f x mean(mobility of the face's cells) (incomp_tpfa.py:223-236) is
+        what the device's harmonic mean returns when each side carries
+        T_f x mean x (cells on the face) / own mobility."""
+        g = self.g
+        T_face = np.asarray(T_face, float).ravel()
+        if T_face.size != g.num_faces:
+            raise ValueError(f"use_trans: one transmissibility per interface ({g.num_faces}) expected, "
+                             f"got {T_face.size}")
+        hf_face = g.cell_faces.ravel()
+        hf_cell = np.repeat(np.arange(g.num_cells), g.cell_faces.shape[1])
+        count = (np.asarray(g.neighbors) >= 0).sum(axis=1)
+        fmob = np.bincount(hf_face, totmob[hf_cell], minlength=g.num_faces) / count
+        ft = T_face * fmob
+        return ft[hf_face] * count[hf_face] / totmob[hf_cell]
 
 
 _handles = {}
@@ -242,11 +334,47 @@ def compute_trans(g: Grid, rock: Rock, device=None) -> np.ndarray:
     return _handle(g, device).compute_trans(rock).cpu().numpy()
 
 
-def incomp_tpfa(state: State, g: Grid, T, fluid: IncompFluid, bc=None, src=None, wells=None, gravity=None,
-                device=None) -> State:
+def incomp_tpfa_ad(state: State, g: Grid, T, fluid: IncompFluid, bc=None, src=None, wells=None, gravity=None,
+                   device=None) -> State:
     """incomp_tpfa_ad (incomp_tpfa.py:252-389): a new state with pressure,
-    flux, facePressure and wellSol."""
+    flux, facePressure and wellSol; `gravity` None means none."""
     return _handle(g, device).solve(state, T, fluid, bc=bc, src=src, wells=wells, gravity=gravity)
+
+
+def incomp_tpfa(state: State, G: Grid, T, fluid: IncompFluid, bc=None, src=None, wells=None, gravity=None, *,
+                W=None, bcp=None, LinSolve=None, MatrixOutput: bool = False, Verbose: bool = False,
+                condition_number: bool = False, pc_form: str = "nonwetting", reduce: bool = False,
+                use_trans: bool = False, device=None) -> State:
+    """incomp_tpfa (incomp_tpfa.py:55-211): a new state with pressure, flux,
+    facePressure and wellSol, and with MatrixOutput (or reduce) the system
+    A, rhs (the wells' Schur complement with reduce).  `gravity` defaults to
+    the process-wide setting (:mod:`pymrst_amd.gravity`), which also sets
+    the wells' hydrostatic drops, as in the reference.  `W` is the legacy
+    name of `wells`; `bcp` adds periodic pressure jumps across its faces;
+    `LinSolve(A, b) -> x` replaces the device solve; `use_trans` takes one
+    transmissibility per interface.  Assembly and reconstruction run on the
+    device in every case."""
+    from .gravity import current_gravity
+    if W is not None:
+        if wells is not None:
+            raise ValueError("give the wells as `wells` or as `W`, not both")
+        wells = W
+    if getattr(fluid, "pc", None) is not None:
+        raise NotImplementedError("capillary pressure is not supported by the device incompTPFA")
+    gvec = current_gravity() if gravity is None else np.resize(np.asarray(gravity, float), 3)
+    if not MatrixOutput and bc is None and src is None and bcp is None and not wells and not np.any(gvec):
+        import warnings
+        warnings.warn("No external driving forces present in model--state remains unchanged.")
+    out = _handle(G, device).solve(state, T, fluid, bc=bc, src=src, wells=wells, gravity=gvec, bcp=bcp,
+                                   lin_solve=LinSolve, matrix_output=MatrixOutput, reduce=reduce,
+                                   use_trans=use_trans, condition_number=condition_number,
+                                   well_gravity=float(np.linalg.norm(current_gravity())))
+    if Verbose:
+        print(f"incompTPFA: {G.num_cells} cells, {len(wells or [])} wells")
+    return out
+
+
+incompTPFA = incomp_tpfa
 
 
 def gravity_vector(dim: int = 3) -> np.ndarray:

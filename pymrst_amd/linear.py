@@ -14,8 +14,12 @@ equations and the control equation per well).  The device system is
 cell-major: B x B blocks per cell pair (any pattern: the wellbore mixture's
 couplings between perforated cells become ordinary blocks), then B + 1
 unknowns per well, coupled to the cells through per-perforation blocks.
-Unlike the reference, a model that lacks what the pressure stage needs is
-an error here, not a direct solve on the host.
+A model that lacks what the pressure stage needs gets the reference's loud
+fallback to a direct solve (linear_solvers.py:426-461), run on the device
+(dense LU, up to DIRECT_MAX unknowns) instead of on the host.  The AD-OO
+seam solve_linear_problem(problem, model) -> (dx, result, report)
+(linear_solver_ad.py:832-893) and the elliptic-solver injection
+(linear_solvers.py:124-131, 253-256) sit on the same device solve.
 """
 from __future__ import annotations
 
@@ -110,6 +114,30 @@ class DeviceSystem:
         _lib.check(self.lib.pmx_sys_matvec(self.h, _lib.ptr(x), _lib.ptr(y)), "pmx_sys_matvec")
         return y
 
+    def set_pressure_solver(self, fn):
+        """fn(b: ndarray) -> x: the pressure stage of every preconditioner
+        application (None: back to the device AMG)."""
+        if fn is None:
+            self._pcb = None
+            _lib.check(self.lib.pmx_sys_set_pressure_solver(self.h, None, None), "pmx_sys_set_pressure_solver")
+            return
+        errors = []
+
+        def call(_user, n, b, x):
+            try:
+                bv = np.ctypeslib.as_array(b, shape=(n,)).copy()
+                np.ctypeslib.as_array(x, shape=(n,))[:] = np.asarray(fn(bv), dtype=float).ravel()
+                return 0
+            except Exception as e:          # reported as a failed solve
+                errors.append(e)
+                return 1
+
+        proto = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double))
+        self._pcb = (proto(call), errors)       # kept alive while the library holds it
+        _lib.check(self.lib.pmx_sys_set_pressure_solver(self.h, ctypes.cast(self._pcb[0], ctypes.c_void_p), None),
+                   "pmx_sys_set_pressure_solver")
+
     def solve(self, opts, w, rhs):
         import torch
         x = torch.zeros_like(rhs)
@@ -117,6 +145,9 @@ class DeviceSystem:
         wv = np.ascontiguousarray(w, dtype=np.float64)
         rc = self.lib.pmx_sys_solve(self.h, ctypes.byref(opts), _lib.ptr(wv), _lib.ptr(rhs), _lib.ptr(x),
                                     ctypes.byref(info))
+        pcb = getattr(self, "_pcb", None)
+        if rc < 0 and pcb is not None and pcb[1]:
+            raise pcb[1][-1]
         if rc < 0:
             _lib.check(rc, "pmx_sys_solve")
         return x, rc, info
@@ -197,12 +228,60 @@ def split_reference_jacobian(jac, nc: int, B: int, keys_hint=None, pkeys_hint=No
     return SplitSystem(rowptr, col, val, perf_ptr, perf_cells, Bp[:nperf], Cp[:nperf], Dp[:nw], dev, keys, pk)
 
 
+DIRECT_MAX = 20000
+
+
+def device_direct_solve(jac, rhs) -> np.ndarray:
+    """jac x = rhs by a dense LU on the GPU (the reference's direct solve,
+    on the device); up to DIRECT_MAX unknowns."""
+    import torch
+    n = jac.shape[0]
+    if jac.shape[0] != jac.shape[1]:
+        raise ValueError(f"a direct solve needs a square matrix, got {jac.shape}")
+    if n > DIRECT_MAX:
+        raise ValueError(f"a direct solve of {n} unknowns is beyond the device's dense limit ({DIRECT_MAX}); "
+                         "give the model (.G, .n_phases, .fluid) for the CPR path")
+    dev = torch.device("cuda")
+    A = torch.as_tensor(sp.csr_array(jac).toarray(), dtype=torch.float64, device=dev)
+    b = torch.as_tensor(np.asarray(rhs, dtype=float).ravel(), device=dev)
+    return torch.linalg.solve(A, b).cpu().numpy()
+
+
+def ruiz_equilibrate(a, sweeps: int = 3):
+    """Row and column scalings dr, dc with dr A dc of unit max-norm rows and
+    columns after `sweeps` alternating passes (the reference equilibrates
+    the pressure system so, linear_solvers.py:205-224)."""
+    a = sp.csr_array(a, dtype=float)
+    dr, dc = np.ones(a.shape[0]), np.ones(a.shape[1])
+    for _ in range(sweeps):
+        b = sp.diags_array(dr) @ a @ sp.diags_array(dc)
+        rmax = np.sqrt(abs(b).max(axis=1).toarray().ravel())
+        rmax[rmax == 0.0] = 1.0
+        dr = dr / rmax
+        b = sp.diags_array(dr) @ a @ sp.diags_array(dc)
+        cmax = np.sqrt(abs(b).max(axis=0).toarray().ravel())
+        cmax[cmax == 0.0] = 1.0
+        dc = dc / cmax
+    return (sp.diags_array(dr) @ a @ sp.diags_array(dc)).tocsr(), dr, dc
+
+
 def solve_reference_system(solver, jac, rhs, model, state) -> np.ndarray:
     """CPRSolver.solve's body: reorder, solve on the device, order back."""
+    import warnings
+
     import torch
     missing = unmet_requirements(model)
     if missing:
-        raise ValueError("the device CPR needs " + "; ".join(missing) + " (there is no host direct fallback)")
+        # the reference's fallback: loud when a model was given and falls
+        # short, quiet when none was (the documented direct path)
+        if model is not None:
+            warnings.warn(f"CPR fell back to a direct solve: {'; '.join(missing)}. The answer is correct and "
+                          "the cost is that of a full factorisation - set the missing attributes to get the "
+                          "preconditioned path.", RuntimeWarning, stacklevel=3)
+        solver.last_iterations = 0
+        solver.last_fallback = tuple(missing)
+        return device_direct_solve(jac, rhs)
+    solver.last_fallback = ()
     if state is None:
         raise ValueError("the device CPR needs the state (its mean pressure sets the weights)")
     nc = int(model.G.cells.num)
@@ -222,13 +301,117 @@ def solve_reference_system(solver, jac, rhs, model, state) -> np.ndarray:
         solver._sys = sysd
     sysd.set_values(val, Bp, Cp, Dp)
     w = pressure_weights(model, state, B)
+    elliptic = getattr(solver, "elliptic_solver", None)
+    if elliptic is not None:
+        # the caller's pressure solver on A_p = W J P, equilibrated as the
+        # reference equilibrates it before handing it over
+        a = sp.csr_array(jac)
+        a_p = sum(w[i] * a[i * nc:(i + 1) * nc, :nc] for i in range(B))
+        a_eq, dr, dc = ruiz_equilibrate(a_p)
+        sysd.set_pressure_solver(lambda b: dc * np.asarray(elliptic.solve_linear_system(a_eq, dr * b)).ravel())
     rhs = np.asarray(rhs, dtype=float).ravel()
     rd = np.empty_like(rhs)
     rd[dev] = rhs
-    x, rc, info = sysd.solve(solver.options(), w, torch.as_tensor(rd, device=sysd.device))
+    try:
+        x, rc, info = sysd.solve(solver.options(), w, torch.as_tensor(rd, device=sysd.device))
+    finally:
+        if elliptic is not None:
+            sysd.set_pressure_solver(None)
     solver.last_iterations = int(info.iterations)
     solver.history.append((int(info.iterations), float(info.rel_residual)))
     if rc == 1:
         raise RuntimeError(f"CPR-FGMRES did not converge ({info.iterations} iterations, relative residual "
                            f"{info.rel_residual:.2e})")
     return x.cpu().numpy()[dev]
+
+
+# -- the AD-OO seam (linear_solver_ad.py:55-140, 807-935) -----------------------
+class LinearizedProblemAD:
+    """A Newton system A dx = -b with its primary-variable layout: A, the
+    residual b, var_names / var_sizes (the column blocks), state, dt; or
+    built from AD equations (each with .val and .jac, one Jacobian block per
+    primary variable) with from_equations."""
+
+    def __init__(self, A=None, b=None, var_names=None, var_sizes=None, state=None, dt=None):
+        self.A = A
+        self.b = np.zeros(0) if b is None else np.asarray(b, dtype=float).ravel()
+        self.var_names = None if var_names is None else list(var_names)
+        self.var_sizes = None if var_sizes is None else [int(v) for v in var_sizes]
+        self.state, self.dt = state, dt
+        self.equations, self.types, self.equation_names = [], [], []
+        self.num_vars = None
+
+    @classmethod
+    def from_equations(cls, equations, types, names, primary_vars, state=None, dt=0.0):
+        prob = cls(state=state, dt=dt, var_names=[primary_vars] if isinstance(primary_vars, str) else primary_vars)
+        prob.equations, prob.types, prob.equation_names = list(equations), list(types), list(names)
+        return prob
+
+    def assemble_system(self):
+        """A and the residual b from the equations (combine_equations)."""
+        if self.A is None and self.equations:
+            blocks = [e for e in self.equations]
+            self.b = np.concatenate([np.asarray(getattr(e, "val", e), dtype=float).ravel() for e in blocks])
+            self.A = sp.vstack([sp.hstack(list(e.jac), format="csr") for e in blocks], format="csr")
+        return self
+
+    def num_eq(self) -> int:
+        return len(self.equations)
+
+
+def problem_system(problem):
+    """(A, rhs) of a linearized problem, assembling it when it holds only its
+    equations; rhs is minus the residual (a LinearizedProblemAD stores the
+    residual, an MRST LinearizedProblem the right-hand side)."""
+    assemble = getattr(problem, "assemble_system", None)
+    if getattr(problem, "A", None) is None and callable(assemble):
+        out = assemble()
+        problem = out if out is not None else problem
+    A, b = problem.A, np.asarray(problem.b, dtype=float).ravel()
+    if A is None:
+        raise ValueError("the linearized problem has neither a matrix nor equations")
+    rhs = -b if type(problem).__name__ == "LinearizedProblemAD" else b
+    return sp.csr_array(A), rhs
+
+
+def split_increment(problem, result):
+    """The increment as one block per primary variable (MRST's
+    storeIncrements): widths from the problem's num_vars, else from its first
+    equation's Jacobian blocks, else its var_sizes; whole otherwise."""
+    result = np.asarray(result, dtype=float).ravel()
+    eqs = [e for e in getattr(problem, "equations", []) if hasattr(e, "jac")]
+    num_vars = getattr(problem, "num_vars", None)
+    if eqs and num_vars is not None and sum(num_vars) == sum(j.shape[1] for j in eqs[0].jac):
+        widths = list(num_vars)
+    elif eqs:
+        widths = [j.shape[1] for j in eqs[0].jac]
+    elif getattr(problem, "var_sizes", None) and getattr(problem, "var_names", None):
+        widths = list(problem.var_sizes)
+    elif result.size == 0:
+        return []
+    else:
+        return [result]
+    edges = np.concatenate([[0], np.cumsum(widths)]).astype(np.int64)
+    return [result[edges[i]:edges[i + 1]] for i in range(len(widths))]
+
+
+def solve_linear_problem(solver, problem, model=None):
+    """(dx, result, report) as the reference's solveLinearProblem returns:
+    the device solve of A dx = -residual, split per primary variable, and a
+    report with MRST's fields."""
+    import time
+    t0 = time.perf_counter()
+    A, rhs = problem_system(problem)
+    if not np.all(np.isfinite(rhs)):
+        raise ValueError("Linear system rhs must have finite entries.")
+    t_prep = time.perf_counter() - t0
+    state = getattr(problem, "state", None)
+    result = np.asarray(solver.solve(A, rhs, model=model, state=state, dt=getattr(problem, "dt", None)),
+                        dtype=float).ravel()
+    t_solve = time.perf_counter() - t0 - t_prep
+    res = float(np.linalg.norm(A @ result - rhs) / max(np.linalg.norm(rhs), np.finfo(float).tiny))
+    total = time.perf_counter() - t0
+    report = {"Iterations": int(getattr(solver, "last_iterations", 0)), "Residual": res, "SolverTime": total,
+              "LinearSolutionTime": t_solve, "PreparationTime": t_prep,
+              "PostProcessTime": total - t_solve - t_prep, "Converged": True}
+    return split_increment(problem, result), result, report

@@ -115,17 +115,43 @@ class CPRSolver:
     local_sweeps: int = 4
     use_graphs: bool = True
     coarse_block_rows: int = 0
+    # a pressure-stage solver with solve_linear_system(A_p, r) -> x, called on
+    # the equilibrated A_p = W J P in every preconditioner application
+    # instead of the device AMG (linear_solvers.py:124-131, 253-256)
+    elliptic_solver: object = None
     last_iterations: int = 0
+    last_fallback: tuple = ()
     history: list = field(default_factory=list)
 
     def solve(self, jac, rhs, model=None, state=None, dt=None):
         """The reference's seam (linear_solvers.py:426-545): solve the
         caller's Jacobian (SciPy sparse, the reference's variable-blocked
         order) by CPR-FGMRES on the device; returns the increment as a NumPy
-        array.  Non-convergence raises RuntimeError; a model without
-        .G, .n_phases and .fluid.{disgas, vapoil, b} raises ValueError."""
+        array.  Non-convergence raises RuntimeError.  A model without .G,
+        .n_phases and .fluid.{disgas, vapoil, b} gets the reference's loud
+        fallback (a RuntimeWarning) to a direct solve, on the device; no
+        model, the same solve without the warning."""
         from .linear import solve_reference_system
         return solve_reference_system(self, jac, rhs, model, state)
+
+    def solve_linear_system(self, A, b):
+        """A x = b without a model: the direct solve, on the device
+        (linear_solver_ad.py:807)."""
+        from .linear import device_direct_solve
+        self.last_iterations = 0
+        return device_direct_solve(A, b)
+
+    def solve_linear_problem(self, problem, model=None):
+        """The AD-OO seam (linear_solver_ad.py:832-893): (dx per primary
+        variable, the increment, a report with MRST's fields)."""
+        from .linear import solve_linear_problem
+        return solve_linear_problem(self, problem, model)
+
+    def get_description(self) -> str:
+        return f"device CPR-FGMRES (FP{self.precision} preconditioner)"
+
+    solveLinearSystem = solve_linear_system
+    solveLinearProblem = solve_linear_problem
 
     def options(self) -> _lib.SolverOptions:
         o = _lib.solver_defaults()

@@ -120,3 +120,104 @@ def test_incomp_field_case_matches_oracle():
     q_in = st.wellSol[0].flux.sum()
     q_out = st.wellSol[1].flux.sum()
     assert abs(q_in + q_out) <= 1e-9 * abs(q_in)
+
+
+# -- the rest of incomp_tpfa's surface (incomp_tpfa.py:55-63, 172-184) -----------
+def _wells_case():
+    """The wells-and-gravity case of MRST's phase-2 replay (10x10x3)."""
+    pa, pi, g, rock, solver = _device_case([10, 10, 3], [1000.0, 1000.0, 30.0],
+                                           np.tile(np.array([100.0, 100.0, 10.0]) * og.MILLI * og.DARCY,
+                                                   (300, 1)), 0.3)
+    gv = pi.gravity_vector(3)
+    w = pa.add_well(None, g, rock, [0, 100, 200], "rate", 5e-3, compi=[1.0], gravity=gv)
+    w = pa.add_well(w, g, rock, [99, 199, 299], "bhp", 100 * og.BARSA, compi=[1.0], gravity=gv)
+    go = og.cart_grid([10, 10, 3], [1000.0, 1000.0, 30.0])
+    ro = og.make_rock(go, np.tile(np.array([100.0, 100.0, 10.0]) * og.MILLI * og.DARCY, (300, 1)), 0.3)
+    wo = og.add_well(None, go, ro, [0, 100, 200], "rate", 5e-3, compi=[1.0], gravity=gv)
+    wo = og.add_well(wo, go, ro, [99, 199, 299], "bhp", 100 * og.BARSA, compi=[1.0], gravity=gv)
+    return pa, pi, g, rock, w, go, ro, wo, gv
+
+
+def test_matrix_output_and_lin_solve_match_the_oracle():
+    """MatrixOutput returns the oracle's assembled system (null-space pin and
+    well rows included) to 1e-13; LinSolve(A, b) on it gives the device
+    solve's state; reduce returns the wells' Schur complement with the same
+    pressures.  Gravity from the process-wide setting (on), as the
+    reference's default."""
+    import scipy.sparse.linalg as spla
+    from pymrst_amd.gravity import isolated_gravity
+    pa, pi, g, rock, w, go, ro, wo, gv = _wells_case()
+    T = pi.compute_trans(g, rock)
+    fluid = pi.init_single_fluid(MU1, 1000.0)
+    ref = o_incomp(go, og.compute_trans(go, ro), np.full((300, 1), 1 / MU1), np.array([1000.0]), wells=wo,
+                   gravity=gv)
+    calls = []
+
+    def direct(A, b):
+        calls.append(A.shape)
+        return spla.spsolve(A.tocsc(), b)
+
+    with isolated_gravity("reset", "on"):
+        dev = pi.incomp_tpfa(pi.init_res_sol(g, 0.0), g, T, fluid, W=w, MatrixOutput=True)
+        host = pi.incomp_tpfa(pi.init_res_sol(g, 0.0), g, T, fluid, wells=w, LinSolve=direct)
+        red = pi.incomp_tpfa(pi.init_res_sol(g, 0.0), g, T, fluid, wells=w, reduce=True, LinSolve=direct)
+    Ad, Ao = dev.A.toarray(), ref.A.toarray()
+    assert Ad.shape == Ao.shape == (302, 302)
+    assert np.abs(Ad - Ao).max() <= 1e-13 * np.abs(Ao).max()
+    assert close(dev.rhs, ref.rhs, 1e-13)[0]
+    assert calls == [(302, 302), (300, 300)]
+    assert red.A.shape == (300, 300) and red.rhs.size == 300
+    for st in (dev, host, red):
+        assert close(st.pressure, ref.pressure, 1e-9)[0]
+        assert close(st.flux, ref.flux, 1e-9)[0]
+        for a, b in zip(st.wellSol, ref.well_flux):
+            assert close(a.flux, b, 1e-9)[0]
+
+
+def test_global_gravity_default_and_use_trans():
+    """gravity=None reads the process-wide setting: on equals the explicit
+    vector, off equals no gravity; use_trans with the harmonic face
+    transmissibilities gives the one-sided solve (incomp_tpfa.py:544-559)."""
+    from pymrst_amd.gravity import isolated_gravity
+    pa, pi, g, rock, w, go, ro, wo, gv = _wells_case()
+    T = pi.compute_trans(g, rock)
+    fluid = pi.init_single_fluid(MU1, 1000.0)
+    st0 = pi.init_res_sol(g, 0.0)
+    with isolated_gravity("reset", "on"):
+        on = pi.incomp_tpfa(st0, g, T, fluid, wells=w)
+        explicit = pi.incomp_tpfa(st0, g, T, fluid, wells=w, gravity=gv)
+    with isolated_gravity("reset", "off"):
+        off = pi.incomp_tpfa(st0, g, T, fluid, wells=w)
+        half = pi.incomp_tpfa(st0, g, T, fluid, wells=w)
+        t = np.asarray(T)
+        Tf = 1.0 / np.bincount(g.cell_faces.ravel(), 1.0 / t, minlength=g.num_faces)
+        face = pi.incomp_tpfa(st0, g, Tf, fluid, wells=w, use_trans=True)
+    assert np.array_equal(on.pressure, explicit.pressure)
+    assert not close(on.pressure, off.pressure, 1e-6)[0]
+    assert close(face.pressure, half.pressure, 1e-9)[0]
+    internal = np.all(g.neighbors >= 0, axis=1)
+    assert np.abs(face.flux[internal] - half.flux[internal]).max() <= 1e-9 * np.abs(half.flux).max()
+
+
+def test_periodic_pressure_jump_adds_to_the_driving_force():
+    """A pressure jump d across one internal face of a 1-D chain between two
+    Dirichlet ends adds to the driving force: the chain's flux becomes
+    (p_L - p_R + d) / R with R = (p_L - p_R) / q(0) from the run without it."""
+    from types import SimpleNamespace
+    pa, pi, g, rock, solver = _device_case([12, 1, 1], [120.0, 10.0, 10.0], 100 * og.MILLI * og.DARCY, 0.2)
+    T = pi.compute_trans(g, rock)
+    fluid = pi.init_single_fluid(MU1, 1000.0)
+    bd = np.any(g.neighbors == -1, axis=1)
+    fc = g.face_centroids
+    left, right = np.flatnonzero(bd & (fc[:, 0] < 1e-10)), np.flatnonzero(bd & (fc[:, 0] > 120 - 1e-10))
+    bc = pi.add_bc(pi.add_bc(None, left, "pressure", 200 * og.BARSA), right, "pressure", 100 * og.BARSA)
+    xfaces = np.flatnonzero(np.all(g.neighbors >= 0, axis=1) & (np.abs(fc[:, 1] - 5.0) < 1e-9)
+                            & (np.abs(fc[:, 2] - 5.0) < 1e-9))
+    jump_face = xfaces[5]
+    d = 30 * og.BARSA
+    st0 = pi.init_res_sol(g, 0.0)
+    q0 = pi.incomp_tpfa(st0, g, T, fluid, bc=bc, gravity=np.zeros(3)).flux[jump_face]
+    q1 = pi.incomp_tpfa(st0, g, T, fluid, bc=bc, gravity=np.zeros(3),
+                        bcp=SimpleNamespace(face=np.array([jump_face]), value=np.array([d]))).flux[jump_face]
+    R = 100 * og.BARSA / q0
+    assert q1 == pytest.approx((100 * og.BARSA + d) / R, rel=1e-9)

@@ -112,3 +112,52 @@ def test_device_system_is_reused_while_the_pattern_fits_and_failure_raises():
     tight = CPRSolver(rtol=1e-14, maxiter=1, restart=2, use_pressure_stage=False, local_stage="block_jacobi")
     with pytest.raises(RuntimeError, match="did not converge"):
         tight.solve(J1, -r1, model=model, state=st1)
+
+
+def test_ad_oo_seam_elliptic_injection_and_direct_fallback():
+    """solve_linear_problem(problem, model) -> (dx, result, report) on the
+    oil-water system equals the direct solve, split per primary variable;
+    an injected elliptic solver (a SciPy direct solve of the equilibrated
+    pressure system, the reference's AMGCL slot) replaces the device AMG
+    and still gives the direct solution; a model missing the CPR
+    attributes warns and solves directly on the device, and no model
+    solves directly without a warning."""
+    import warnings
+    import scipy.sparse as sp
+    from pymrst_amd.linear import LinearizedProblemAD
+    from pymrst_amd.ow import CPRSolver
+    J, r, model, state = _ow_system(11)
+    x_ref = spla.spsolve(J.tocsc(), -r)
+    nc = model.G.cells.num
+    nw = (J.shape[0] - 2 * nc) // 3
+    prob = LinearizedProblemAD(A=J, b=r, var_names=["pressure", "sW", "qWs", "qOs", "bhp"],
+                               var_sizes=[nc, nc, nw, nw, nw], state=state)
+    dx, result, report = CPRSolver(rtol=1e-10).solve_linear_problem(prob, model)
+    assert np.abs(result - x_ref).max() <= 1e-7 * np.abs(x_ref).max()
+    assert [d.size for d in dx] == [nc, nc, nw, nw, nw] and np.array_equal(np.concatenate(dx), result)
+    assert report["Converged"] and report["Iterations"] > 0 and report["Residual"] <= 1e-9
+
+    class Elliptic:
+        def __init__(self):
+            self.shapes = []
+
+        def solve_linear_system(self, A, b):
+            self.shapes.append(A.shape)
+            d = A.diagonal()
+            off = (A - sp.diags_array(d)).tocoo()
+            assert (d > 0).all() and (off.data <= 1e-12).all()      # W J P is an M-matrix
+            return spla.spsolve(sp.csc_matrix(A), b)
+
+    ell = Elliptic()
+    solver = CPRSolver(rtol=1e-10, elliptic_solver=ell)
+    x = solver.solve(J, -r, model=model, state=state)
+    assert np.abs(x - x_ref).max() <= 1e-7 * np.abs(x_ref).max()
+    assert len(ell.shapes) >= solver.last_iterations >= 1 and set(ell.shapes) == {(nc, nc)}
+    with pytest.warns(RuntimeWarning, match="fell back to a direct solve"):
+        xf = CPRSolver().solve(J, -r, model=SimpleNamespace(G=model.G), state=state)
+    assert np.abs(xf - x_ref).max() <= 1e-10 * np.abs(x_ref).max()
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        xn = CPRSolver().solve(J, -r)
+        xs = CPRSolver().solve_linear_system(J, -r)
+    assert np.abs(xn - x_ref).max() <= 1e-10 * np.abs(x_ref).max() and np.array_equal(xn, xs)

@@ -94,11 +94,25 @@ def test_black_oil_jacobian_split_reassembles_exactly():
     _check(J, nc, 3)
 
 
-def test_missing_model_attributes_are_an_error_not_a_host_solve():
+def test_requirements_and_the_ad_oo_problem_plumbing():
+    """The reference's requirement list (linear_solvers.py:192-224), the
+    AD-OO problem's right-hand side (minus the stored residual) and its split
+    of the increment per primary variable (storeIncrements), from the
+    declared sizes and from AD equations' Jacobian blocks."""
     import scipy.sparse as sp
-    from pymrst_amd.ow import CPRSolver
-    J = sp.eye(6, format="csr")
-    with pytest.raises(ValueError, match="no host direct fallback"):
-        CPRSolver().solve(J, np.ones(6), model=SimpleNamespace(G=None))
-    with pytest.raises(ValueError, match="no model"):
-        CPRSolver().solve(J, np.ones(6))
+    from pymrst_amd.linear import LinearizedProblemAD, problem_system, split_increment, unmet_requirements
+    assert unmet_requirements(None) == ["no model was supplied"]
+    assert unmet_requirements(SimpleNamespace(G=None, n_phases=2)) == ["model has no .fluid"]
+    fl = SimpleNamespace(disgas=False, b=None)
+    assert unmet_requirements(SimpleNamespace(G=None, n_phases=2, fluid=fl)) == ["model.fluid has no .vapoil"]
+    A = sp.random(5, 5, density=0.5, random_state=0) + sp.eye(5)
+    p = LinearizedProblemAD(A=A, b=np.arange(5.0), var_names=["p", "s"], var_sizes=[2, 3])
+    M, rhs = problem_system(p)
+    assert np.array_equal(rhs, -np.arange(5.0)) and np.array_equal(M.toarray(), A.toarray())
+    assert [x.tolist() for x in split_increment(p, np.arange(5.0))] == [[0.0, 1.0], [2.0, 3.0, 4.0]]
+    eq = SimpleNamespace(val=np.ones(2), jac=[sp.eye(2, format="csr"), sp.csr_array((2, 1))])
+    eq2 = SimpleNamespace(val=np.zeros(1), jac=[sp.csr_array((1, 2)), sp.eye(1, format="csr")])
+    q = LinearizedProblemAD.from_equations([eq, eq2], ["cell", "cell"], ["a", "b"], ["x", "y"])
+    M, rhs = problem_system(q)
+    assert M.shape == (3, 3) and np.array_equal(rhs, [-1.0, -1.0, 0.0])
+    assert [x.size for x in split_increment(q, np.zeros(3))] == [2, 1]
