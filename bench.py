@@ -53,12 +53,17 @@ OW_CPR = {"local_sweeps": 5, "jacobi_omega": 1.0}
 BO_CPR = {"local_sweeps": 3}
 
 
-def run_form(case, precision, steps, warmup, rtol):
+def run_form(case, precision, steps, warmup, rtol, local_stage=None):
+    """Time `steps` report steps of the schedule: passes over it from the
+    initial state, the last one cut short, so any step count is exact."""
     import torch
     from pymrst_amd.ow import CPRSolver, simulate_schedule_ad
     from pymrst_amd.workloads import build_model
+    cpr = dict(OW_CPR)
+    if local_stage is not None:
+        cpr["local_stage"] = local_stage
     # no per-solve log lines inside the timed region (the observer reports each report step)
-    solver = CPRSolver(rtol=rtol, precision=precision, verbose=0, **OW_CPR)
+    solver = CPRSolver(rtol=rtol, precision=precision, verbose=0, **cpr)
     t_setup = time.perf_counter()
     model, s0 = build_model(case, solver)
     print(f"[bench] FP{precision}: model set up in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr,
@@ -70,20 +75,77 @@ def run_form(case, precision, steps, warmup, rtol):
             print(f"[bench] FP{precision} {event} {payload}", file=sys.stderr, flush=True)
 
     if warmup > 0:
-        simulate_schedule_ad(s0, model, dts[:warmup], keep_device=True, observer=progress)
+        simulate_schedule_ad(s0, model, dts[:min(warmup, dts.size)], keep_device=True, observer=progress)
     model.reset()
     solver.history.clear()
     model.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    states, its = simulate_schedule_ad(s0, model, dts[:steps], keep_device=True, observer=progress)
+    states, its, done = [], [], 0
+    while done < steps:
+        n = min(steps - done, dts.size)
+        if done:
+            model.reset()          # a new pass over the schedule starts from the initial state
+        st, it = simulate_schedule_ad(s0, model, dts[:n], keep_device=True, observer=progress)
+        states += st
+        its.append(it)
+        done += n
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    its = np.concatenate(its)
     prof = model.profile_read()
     model.profile(False)
     p_hist = [s.p.cpu().numpy() for s in states]
     return dict(model=model, its=its, wall=wall, p_hist=p_hist, prof=prof,
                 lin_its=sum(h[0] for h in solver.history), n_solves=len(solver.history))
+
+
+def incomp_leg(reps: int = 5):
+    """The incompTPFA pressure solve (32x32x8, lognormal permeability,
+    gravity, rate injector and bhp producer columns): device assembly, AMG-
+    FGMRES solve and reconstruction timed over `reps` solves, against the
+    oracle's assembly with SciPy's SuperLU solve (the reference's
+    spsolve, incomp_tpfa.py:172-184), single thread, on the same inputs."""
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle.grid as og
+    from oracle.incomp import incomp_tpfa as o_incomp
+    from pymrst_amd import incomp as pi
+    from pymrst_amd.workloads import incomp_case
+    c = incomp_case()
+    g, rock, wells, fluid, gv = c["grid"], c["rock"], c["wells"], c["fluid"], c["gravity"]
+    solver = pi.IncompTPFA(g)
+    T = solver.compute_trans(rock)
+    st0 = pi.init_res_sol(g, 0.0)
+    solver.solve(st0, T, fluid, wells=wells, gravity=gv)          # warm-up: allocations, AMG hierarchy
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = solver.solve(st0, T, fluid, wells=wells, gravity=gv)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / reps
+    go = og.cart_grid(c["dims"], c["phys"])
+    ro = og.make_rock(go, c["perm"], c["poro"])
+    wo = None
+    for cells, kind, val in c["well_specs"]:
+        wo = og.add_well(wo, go, ro, cells, kind, val, compi=[1.0], gravity=gv)
+    to = og.compute_trans(go, ro)
+    mob = np.full((go.num_cells, 1), 1.0 / c["mu"])
+    times = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        ref = o_incomp(go, to, mob, np.array([c["rho"]]), wells=wo, gravity=gv)
+        times.append(time.perf_counter() - t1)
+    cpu_s = float(np.median(times))
+    err = float(np.abs(st.pressure - ref.pressure).max() / np.abs(ref.pressure).max())
+    nc = g.num_cells
+    return {"workload": f"incompTPFA {c['dims'][0]}x{c['dims'][1]}x{c['dims'][2]} ({nc} cells), lognormal k, "
+                        "gravity, rate injector + bhp producer columns",
+            "gpu_ms_per_solve": round(gpu_s * 1e3, 3), "fgmres_its": int(solver.last_info.iterations),
+            "max_rel_pressure_diff_vs_oracle": err,
+            "cpu_baseline": {"value": round(cpu_s * 1e3, 3), "unit": "ms per solve", "cores": 1, "kind": "port",
+                             "sample": "the oracle's incompTPFA (assembly, SciPy SuperLU spsolve, flux "
+                                       "reconstruction) on the same inputs, median of 3, 1 thread"}}
 
 
 def cpu_baseline(case_sample, budget_s=25.0):
@@ -405,6 +467,10 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--ensemble-loop", type=int, default=16, help="members timed in the looped plain form")
+    ap.add_argument("--local-stage", choices=("line", "ilu", "block_jacobi"), default=None,
+                    help="CPR local stage of the oil-water legs (default: line Gauss-Seidel)")
+    ap.add_argument("--no-incomp", action="store_true")
+    ap.add_argument("--incomp-only", action="store_true")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
                     help="run only the SpMV and a calibration copy REPS times (for counter runs)")
     args = ap.parse_args()
@@ -430,12 +496,15 @@ def main():
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args.adjoint_fd)}))
         return 0
-    steps = max(1, min(args.steps, case.dts.size))
+    if args.incomp_only:
+        print(json.dumps({"incomp": incomp_leg()}))
+        return 0
+    steps = max(1, args.steps)
     nc = case.num_cells
     forms = {"both": (64, 32), "mixed": (32,), "fp64": (64,)}[args.form]
     res = {}
     for prec in forms:
-        res[prec] = run_form(case, prec, steps, args.warmup, args.rtol)
+        res[prec] = run_form(case, prec, steps, args.warmup, args.rtol, args.local_stage)
         r = res[prec]
         print(f"[bench] FP{prec} preconditioner: {int(r['its'].sum())} Newton its, {r['lin_its']} GMRES its, "
               f"{r['wall']:.3f} s", file=sys.stderr, flush=True)
@@ -468,7 +537,7 @@ def main():
     # stage; it runs inside the replayed preconditioner graph, so it is timed
     # launch by launch in eager applications on the last Newton system
     roof_spmv, roof = roof, None
-    if head["model"].linear_solver.local_stage == "line":
+    if head["model"].linear_solver.local_stage == "line" and nc > 100000:
         lg = head["model"].time_line_solve(20)
         if lg["launches"] > 0:
             avg_s = lg["ms"] / lg["launches"] / 1e3
@@ -487,16 +556,21 @@ def main():
         sample = field_waterflood(dims=(dims[0], max(2, dims[1] // 10), dims[2]))
         cpu = cpu_baseline(sample)
     line = {"metric": "Mcell-Newton-its/s", "value": round(value, 3), "unit": "Mcell-Newton-its/s",
-            "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "n_gpus": 1, "steps": steps, "steps_requested": args.steps, "warmup": args.warmup,
             "ms_per_step": round(head["wall"] / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f64 (FP32 preconditioner)" if 32 in res else "f64",
             "data": "synthetic",
             "config": {"workload": f"oil-water field waterflood {dims[0]}x{dims[1]}x{dims[2]} "
-                                   f"({nc} cells), five-spot, gravity, {steps} x 30 d, CPR-FGMRES rtol {args.rtol}",
-                       "cells": nc, "report_steps": steps, "cpr": dict(OW_CPR)},
+                                   f"({nc} cells), five-spot, gravity, {steps} report steps of 30 d (passes over "
+                                   f"the {case.dts.size}-step schedule from the initial state), CPR-FGMRES rtol "
+                                   f"{args.rtol}",
+                       "cells": nc, "report_steps": steps,
+                       "cpr": dict(OW_CPR, **({"local_stage": args.local_stage} if args.local_stage else {}))},
             "roofline": roof, "roofline_spmv": roof_spmv, "cpu_baseline": cpu}
     line.update(extra)
     line["stream_copy"] = stream_copy()
+    if not args.no_incomp:
+        line["incomp"] = incomp_leg()
     if not args.no_blackoil:
         line["blackoil"] = blackoil_leg(cpu=not args.no_cpu_baseline)
     if not args.no_adjoint:

@@ -39,7 +39,8 @@ typedef struct pmx_solver_options {
   int32_t pairwise_passes;
   int32_t reorthogonalize;
   int32_t verbose;
-  int32_t local_stage;      /* 0: cell block-Jacobi, 1: coloured line Gauss-Seidel */
+  int32_t local_stage;      /* 0: cell block-Jacobi, 1: coloured line Gauss-Seidel, 2: block ILU(0),
+                               level-scheduled (MRST's ILU(0), gmres_ilu_solver_ad.py:119-209) */
   int32_t pressure_cycles;  /* V-cycles per pressure-stage application (stationary iteration) */
   int32_t local_sweeps;     /* Gauss-Seidel sweeps over the line colours */
   int32_t use_graphs;       /* 1: replay the preconditioner as a captured HIP graph */

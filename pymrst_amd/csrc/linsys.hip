@@ -902,7 +902,7 @@ __global__ __launch_bounds__(256) void k_line_r1(int nc, const int* __restrict__
             for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
         }
     }
-    const size_t at = (size_t)pos[row];
+    const size_t at = pos != nullptr ? (size_t)pos[row] : (size_t)row;
 #pragma unroll
     for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
     xpl[at] = xp[row];
@@ -961,7 +961,7 @@ __global__ __launch_bounds__(256) void k_line_r1_sell(int nc, const int* __restr
           for (int x = 0; x < X; ++x) r1[a] -= Um[(size_t)j * B * X + a * X + x] * hp[w * X + x];
       }
   }
-  const size_t at = (size_t)pos[row];
+  const size_t at = pos != nullptr ? (size_t)pos[row] : (size_t)row;
 #pragma unroll
   for (int a = 0; a < B; ++a) r1l[at * B + a] = (T)r1[a];
   xpl[at] = xp[row];
@@ -1303,6 +1303,133 @@ __global__ __launch_bounds__(64) void k_line_gs_tw(int nl, int lc0, const int* _
 }
 
 // ---------------------------------------------------------------------------
+// Block ILU(0) local stage on the Jacobian's block pattern, rows scheduled
+// by levels: a row's level is one past the highest level among its lower
+// (for the factorization and the forward solve) or upper (backward solve)
+// neighbours, so the rows of one level are independent.  One thread per row.
+// ---------------------------------------------------------------------------
+// IKJ elimination of the rows of one level, in place on F (FP64 copy of the
+// blocks): L_ik = A_ik U_kk^{-1} for each lower block in column order, and
+// A_ij -= L_ik U_kj for every j > k the row holds (zero fill); then the
+// inverse of the row's pivot U_ii.
+template <int B>
+__global__ void k_ilu_factor_level(int nr, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                   const int* __restrict__ col, const int* __restrict__ diag,
+                                   double* __restrict__ F, double* __restrict__ Dinv) {
+  constexpr int BB = B * B;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nr) return;
+  const int i = rows[t];
+  const int end = rowptr[i + 1], di = diag[i];
+  for (int q = rowptr[i]; q < di; ++q) {
+    const int k = col[q];
+    double a[B][B], dk[B][B], l[B][B];
+    blk_load<B>(F + (size_t)q * BB, a);
+    blk_load<B>(Dinv + (size_t)k * BB, dk);
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < B; ++m) v += a[r][m] * dk[m][c];
+        l[r][c] = v;
+      }
+    blk_store<B, double>(F + (size_t)q * BB, l);
+    int qi = q + 1;
+    for (int p = diag[k] + 1; p < rowptr[k + 1]; ++p) {
+      const int j = col[p];
+      while (qi < end && col[qi] < j) ++qi;
+      if (qi >= end) break;
+      if (col[qi] != j) continue;
+      double u[B][B], x[B][B];
+      blk_load<B>(F + (size_t)p * BB, u);
+      blk_load<B>(F + (size_t)qi * BB, x);
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          double v = x[r][c];
+#pragma unroll
+          for (int m = 0; m < B; ++m) v -= l[r][m] * u[m][c];
+          x[r][c] = v;
+        }
+      blk_store<B, double>(F + (size_t)qi * BB, x);
+    }
+  }
+  double piv[B][B], inv[B][B];
+  blk_load<B>(F + (size_t)di * BB, piv);
+  small_inverse<B>(piv, inv);
+  blk_store<B, double>(Dinv + (size_t)i * BB, inv);
+}
+
+template <class T>
+__global__ void k_convert(size_t n, const double* __restrict__ a, T* __restrict__ b) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (T)a[i];
+}
+
+// Forward solve of one level, in place: y_i -= sum_{k < i} L_ik y_k.
+template <int B, class T>
+__global__ void k_ilu_lower_level(int nr, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                  const int* __restrict__ col, const int* __restrict__ diag, const T* __restrict__ F,
+                                  T* __restrict__ y) {
+  constexpr int BB = B * B;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nr) return;
+  const int i = rows[t];
+  double v[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) v[a] = (double)y[(size_t)i * B + a];
+  for (int q = rowptr[i]; q < diag[i]; ++q) {
+    const int k = col[q];
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+      for (int b = 0; b < B; ++b) v[a] -= (double)F[(size_t)q * BB + a * B + b] * (double)y[(size_t)k * B + b];
+  }
+#pragma unroll
+  for (int a = 0; a < B; ++a) y[(size_t)i * B + a] = (T)v[a];
+}
+
+// Backward solve of one level, in place: x_i = U_ii^{-1} (y_i - sum_{j > i}
+// U_ij x_j); also z_i = x_i + pc xp_i into the preconditioner output.
+template <int B, class T>
+__global__ void k_ilu_upper_level(int nr, const int* __restrict__ rows, const int* __restrict__ rowptr,
+                                  const int* __restrict__ col, const int* __restrict__ diag, const T* __restrict__ F,
+                                  const T* __restrict__ Dinv, T* __restrict__ y, const T* __restrict__ xp,
+                                  const double* __restrict__ pc, const PrecIO* __restrict__ io) {
+  constexpr int BB = B * B;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nr) return;
+  const int i = rows[t];
+  double v[B], x[B];
+#pragma unroll
+  for (int a = 0; a < B; ++a) v[a] = (double)y[(size_t)i * B + a];
+  for (int q = diag[i] + 1; q < rowptr[i + 1]; ++q) {
+    const int j = col[q];
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+      for (int b = 0; b < B; ++b) v[a] -= (double)F[(size_t)q * BB + a * B + b] * (double)y[(size_t)j * B + b];
+  }
+#pragma unroll
+  for (int a = 0; a < B; ++a) {
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) s += (double)Dinv[(size_t)i * BB + a * B + b] * v[b];
+    x[a] = s;
+  }
+  double* __restrict__ z = io->z;
+  const double xpi = (double)xp[i];
+#pragma unroll
+  for (int a = 0; a < B; ++a) {
+    y[(size_t)i * B + a] = (T)x[a];
+    z[(size_t)i * B + a] = x[a] + pc[a] * xpi;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Aggregation AMG kernels (scalar CSR, value type T)
 // ---------------------------------------------------------------------------
 template <class T>
@@ -1630,6 +1757,13 @@ class CprSolver {
   // repeated pressure cycles: original right-hand side and accumulated solution
   DArr<double> pb64, pacc64;
   DArr<float> pb32, pacc32;
+  // block ILU(0) local stage: rows in level order for the factorization and
+  // forward solve (lower levels) and for the backward solve (upper levels)
+  std::vector<int> ilu_lo_ptr, ilu_up_ptr;
+  DArr<int> ilu_lo_rows, ilu_up_rows;
+  DArr<double> ilu_w64, ilu_dw64;                    // FP64 factorization work: blocks, pivot inverses
+  DArr<double> ilu_f64, ilu_di64, ilu_y64, ilu_xp64;
+  DArr<float> ilu_f32, ilu_di32, ilu_y32, ilu_xp32;
   // a caller's pressure-stage solver (host callback) in place of the AMG
   PressureSolveFn pfn = nullptr;
   void* puser = nullptr;
@@ -1775,6 +1909,35 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   C->off_q.upload(HL.off_q, s);
   C->off_nbr.upload(HL.off_nbr, s);
   C->off_col.upload(HL.off_col, s);
+  // ILU(0) level schedules of the block pattern
+  {
+    const int n = fine.n;
+    std::vector<int> lev(n, 0);
+    int nlo = 0, nup = 0;
+    for (int i = 0; i < n; ++i) {
+      int L = 0;
+      for (int q = fine.rowptr[i]; q < fine.diag[i]; ++q) L = std::max(L, lev[fine.col[q]] + 1);
+      lev[i] = L;
+      nlo = std::max(nlo, L + 1);
+    }
+    auto bucket = [&](int nl, std::vector<int>& ptr, DArr<int>& rows) {
+      ptr.assign(nl + 1, 0);
+      for (int i = 0; i < n; ++i) ++ptr[lev[i] + 1];
+      for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+      std::vector<int> r(n), at(ptr.begin(), ptr.end() - 1);
+      for (int i = 0; i < n; ++i) r[at[lev[i]]++] = i;
+      rows.upload(r, s);
+    };
+    bucket(nlo, C->ilu_lo_ptr, C->ilu_lo_rows);
+    std::fill(lev.begin(), lev.end(), 0);
+    for (int i = n - 1; i >= 0; --i) {
+      int L = 0;
+      for (int q = fine.diag[i] + 1; q < fine.rowptr[i + 1]; ++q) L = std::max(L, lev[fine.col[q]] + 1);
+      lev[i] = L;
+      nup = std::max(nup, L + 1);
+    }
+    bucket(nup, C->ilu_up_ptr, C->ilu_up_rows);
+  }
   C->coarse_n = C->L.back()->n;
   PMX_CHECK(C->coarse_n <= 4096, "coarsest AMG level is too large for the dense solve");
   C->dense.alloc((size_t)C->coarse_n * C->coarse_n);
@@ -1813,6 +1976,10 @@ struct Lv<double> {
   static DArr<double>& yc(CprSolver& c) { return c.yc64; }
   static DArr<double>& r1l(CprSolver& c) { return c.r1l64; }
   static DArr<double>& aoff(CprSolver& c) { return c.aoff64; }
+  static DArr<double>& ilu_f(CprSolver& c) { return c.ilu_f64; }
+  static DArr<double>& ilu_di(CprSolver& c) { return c.ilu_di64; }
+  static DArr<double>& ilu_y(CprSolver& c) { return c.ilu_y64; }
+  static DArr<double>& ilu_xp(CprSolver& c) { return c.ilu_xp64; }
   static DArr<double>& pb(CprSolver& c) { return c.pb64; }
   static DArr<double>& pacc(CprSolver& c) { return c.pacc64; }
 };
@@ -1836,6 +2003,10 @@ struct Lv<float> {
   static DArr<float>& yc(CprSolver& c) { return c.yc32; }
   static DArr<float>& r1l(CprSolver& c) { return c.r1l32; }
   static DArr<float>& aoff(CprSolver& c) { return c.aoff32; }
+  static DArr<float>& ilu_f(CprSolver& c) { return c.ilu_f32; }
+  static DArr<float>& ilu_di(CprSolver& c) { return c.ilu_di32; }
+  static DArr<float>& ilu_y(CprSolver& c) { return c.ilu_y32; }
+  static DArr<float>& ilu_xp(CprSolver& c) { return c.ilu_xp32; }
   static DArr<float>& pb(CprSolver& c) { return c.pb32; }
   static DArr<float>& pacc(CprSolver& c) { return c.pacc32; }
 };
@@ -1893,6 +2064,27 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
                        V::lf_lo(*C).p, V::lf_up(*C).p);
     launch_check();
   }
+  if (o.local_stage == 2) {
+    const size_t nb = (size_t)S.nnzb * B * B, nd = (size_t)S.nc * B * B;
+    C->ilu_w64.alloc(nb);
+    C->ilu_dw64.alloc(nd);
+    PMX_HIP(hipMemcpyAsync(C->ilu_w64.p, S.val.p, sizeof(double) * nb, hipMemcpyDeviceToDevice, s));
+    const int nlev = (int)C->ilu_lo_ptr.size() - 1;
+    for (int l = 0; l < nlev; ++l) {
+      const int r0 = C->ilu_lo_ptr[l], nr = C->ilu_lo_ptr[l + 1] - r0;
+      hipLaunchKernelGGL((k_ilu_factor_level<B>), dim3(grid_for(nr, 64)), dim3(64), 0, s, nr, C->ilu_lo_rows.p + r0,
+                         S.rowptr.p, S.col.p, S.diag.p, C->ilu_w64.p, C->ilu_dw64.p);
+      launch_check();
+    }
+    V::ilu_f(*C).alloc(nb);
+    V::ilu_di(*C).alloc(nd);
+    V::ilu_y(*C).alloc((size_t)S.nc * B);
+    V::ilu_xp(*C).alloc(S.nc);
+    hipLaunchKernelGGL((k_convert<T>), dim3(grid_for(nb)), dim3(256), 0, s, nb, C->ilu_w64.p, V::ilu_f(*C).p);
+    launch_check();
+    hipLaunchKernelGGL((k_convert<T>), dim3(grid_for(nd)), dim3(256), 0, s, nd, C->ilu_dw64.p, V::ilu_di(*C).p);
+    launch_check();
+  }
   if (S.nw > 0) {
     C->tail_inv.alloc((size_t)S.nw * M * M);
     C->hp.alloc((size_t)S.nw * X);
@@ -1907,7 +2099,7 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
       hipLaunchKernelGGL((k_to_sell<T>), dim3(grid_for(a.nnz)), dim3(256), 0, s, a.nnz, a.sl_map.p, V::v(a).p,
                          V::sv(a).p);
       launch_check();
-      if (l == 0 && o.local_stage == 1 && C->nlines > 0) {
+      if (l == 0 && ((o.local_stage == 1 && C->nlines > 0) || o.local_stage == 2)) {
         PMX_CHECK(a.nnz == S.nnzb && a.n == S.nc, "the finest level shares the Jacobian's block pattern");
         V::Jps(*C).alloc(a.sell_slots * B);
         V::Jps(*C).zero(s);
@@ -2111,6 +2303,36 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
     hipLaunchKernelGGL((k_cpr_wells<B, M, X, T>), dim3(S.nw), dim3(64), 0, s, S.nw, S.well_ptr.p, S.perf_cell.p,
                        S.Cp.p, S.V.p, C->tail_inv.p, V::x(f).p, C->io, ncB, C->hp.p, C->pcdev);
     launch_check();
+  }
+  if (o.local_stage == 2) {
+    // r1 = r - J P xp (and the wells' mixture term) in cell order, then the
+    // ILU(0) forward and backward solves level by level
+    const bool mix = S.nw > 0 && S.has_mix && S.npc > 0;
+    if (f.nslice > 0)
+      hipLaunchKernelGGL((k_line_r1_sell<B, X, T>), dim3(grid_for(S.nc)), dim3(256), 0, s, S.nc, f.sl_ptr.p,
+                         f.sl_len.p, f.sl_col.p, V::Jps(*C).p, V::x(f).p, C->io, mix ? S.pc_of.p : nullptr,
+                         S.pc_ptr.p, S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, nullptr, V::ilu_y(*C).p,
+                         V::ilu_xp(*C).p);
+    else
+      hipLaunchKernelGGL((k_line_r1<B, X, T, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                         S.rowptr.p, S.col.p, V::Jp(*C).p, V::x(f).p, C->io, mix ? S.pc_of.p : nullptr, S.pc_ptr.p,
+                         S.pc_perf.p, S.perf_well.p, S.Um.p, C->hp.p, nullptr, V::ilu_y(*C).p, V::ilu_xp(*C).p);
+    launch_check();
+    const int nlo = (int)C->ilu_lo_ptr.size() - 1, nup = (int)C->ilu_up_ptr.size() - 1;
+    for (int l = 0; l < nlo; ++l) {
+      const int r0 = C->ilu_lo_ptr[l], nr = C->ilu_lo_ptr[l + 1] - r0;
+      hipLaunchKernelGGL((k_ilu_lower_level<B, T>), dim3(grid_for(nr, 64)), dim3(64), 0, s, nr,
+                         C->ilu_lo_rows.p + r0, S.rowptr.p, S.col.p, S.diag.p, V::ilu_f(*C).p, V::ilu_y(*C).p);
+      launch_check();
+    }
+    for (int l = 0; l < nup; ++l) {
+      const int r0 = C->ilu_up_ptr[l], nr = C->ilu_up_ptr[l + 1] - r0;
+      hipLaunchKernelGGL((k_ilu_upper_level<B, T>), dim3(grid_for(nr, 64)), dim3(64), 0, s, nr,
+                         C->ilu_up_rows.p + r0, S.rowptr.p, S.col.p, S.diag.p, V::ilu_f(*C).p, V::ilu_di(*C).p,
+                         V::ilu_y(*C).p, V::ilu_xp(*C).p, C->pcdev, C->io);
+      launch_check();
+    }
+    return;
   }
   if (o.local_stage == 1 && C->nlines > 0) {
     const bool mix = S.nw > 0 && S.has_mix && S.npc > 0;

@@ -161,7 +161,7 @@ class CPRSolver:
         o.jacobi_omega, o.coarse_alpha, o.theta = self.jacobi_omega, self.coarse_alpha, self.theta
         o.coarse_size, o.pairwise_passes = self.coarse_size, self.pairwise_passes
         o.reorthogonalize, o.verbose = int(self.reorthogonalize), self.verbose
-        o.local_stage = {"block_jacobi": 0, "line": 1}[self.local_stage]
+        o.local_stage = {"block_jacobi": 0, "line": 1, "ilu": 2}[self.local_stage]
         o.pressure_cycles = self.pressure_cycles
         o.local_sweeps = self.local_sweeps
         o.use_graphs = int(self.use_graphs)

@@ -252,3 +252,28 @@ def build_bo_model(case: BOCase, linear_solver=None, device=None):
                                       linear_solver=linear_solver, device=device)
     st = model.initial_state(case.p_init, case.sw0, case.sg_init, rs0=case.rs0)
     return model, st
+
+
+def incomp_case(dims=(32, 32, 8), phys=(320.0, 320.0, 16.0), seed=0):
+    """The incompTPFA workload: a 32x32x8 grid of 10x10x2 m cells, isotropic
+    cell-wise lognormal permeability (median 100 mD, sigma(ln k) 1.0, seeded),
+    porosity 0.2, one fluid of 1 cP and 1000 kg/m3, a rate injector of
+    1e-3 m3/s in column (0, 0) and a 100 bar producer in column (31, 31), all
+    layers, other boundaries closed, gravity on.  Returns a dict with the
+    product's grid, rock, wells, fluid and the raw inputs (for the oracle)."""
+    from . import grid as G
+    from . import incomp as pi
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    perm = 100 * MILLI * DARCY * np.exp(1.0 * rng.standard_normal(nx * ny * nz))
+    g = G.cart_grid(dims, phys)
+    rock = G.make_rock(g, perm, 0.2)
+    gv = pi.gravity_vector(3)
+    specs = [(nx * ny * np.arange(nz), "rate", 1e-3), (nx - 1 + nx * (ny - 1) + nx * ny * np.arange(nz), "bhp",
+                                                        100 * BARSA)]
+    wells = None
+    for cells, kind, val in specs:
+        wells = G.add_well(wells, g, rock, cells, kind, val, compi=[1.0], gravity=gv)
+    mu = 1e-3
+    return dict(grid=g, rock=rock, wells=wells, fluid=pi.init_single_fluid(mu, 1000.0), gravity=gv, dims=dims,
+                phys=phys, perm=perm, poro=0.2, mu=mu, rho=1000.0, well_specs=specs)
