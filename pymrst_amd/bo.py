@@ -328,7 +328,7 @@ class ThreePhaseBlackOilModelAD:
         self.last_cnv_mb = (cnv, mb)
         return ok, out[9] / nc, out[10] / nc
 
-    def solve_and_update(self, st: DeviceBOState, pmean: float, rsmean: float) -> bool:
+    def solve_and_update(self, st: DeviceBOState, pmean: float, rsmean: float, scale: float = 1.0) -> bool:
         info = _lib.SolveInfo()
         opts = self.linear_solver.options()
         t0 = time.perf_counter()
@@ -342,10 +342,16 @@ class ThreePhaseBlackOilModelAD:
         self.stats["linear"] += int(info.iterations)
         if rc != 0:
             return False
-        P = _lib.ptr
-        _lib.check(self.lib.pmx_bo_update(self.h, P(self._dx), float(self.ds_max), P(st.p), P(st.s), P(st.rs),
-                                          P(st.status), P(st.q), P(st.bhp)), "pmx_bo_update")
+        self.apply_increment(st, scale)
         return True
+
+    def apply_increment(self, st: DeviceBOState, scale: float = 1.0):
+        """update_state(st, scale * (the last Newton increment)), with the
+        variable switch and the flash."""
+        P = _lib.ptr
+        dx = self._dx if scale == 1.0 else self._dx * float(scale)
+        _lib.check(self.lib.pmx_bo_update(self.h, P(dx), float(self.ds_max), P(st.p), P(st.s), P(st.rs),
+                                          P(st.status), P(st.q), P(st.bhp)), "pmx_bo_update")
 
     # -- ministep protocol of pymrst_amd.nonlinear.NonLinearSolver --------------
     def begin_ministep(self, state0: DeviceBOState):
@@ -358,9 +364,9 @@ class ThreePhaseBlackOilModelAD:
         cnv, mb = self.last_cnv_mb
         return ok, {"CNV": float(cnv), "MB": float(mb)}
 
-    def newton_update(self, st: DeviceBOState) -> bool:
+    def newton_update(self, st: DeviceBOState, scale: float = 1.0) -> bool:
         self.stats["newton"] += 1
-        return self.solve_and_update(st, self._ms_pmean, self._ms_rsmean)
+        return self.solve_and_update(st, self._ms_pmean, self._ms_rsmean, scale)
 
     @property
     def last_linear_solve(self):

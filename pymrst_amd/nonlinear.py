@@ -16,9 +16,16 @@ update_state:
     model.linearize(state0, state, dt)       -> (converged, residuals dict)
     model.newton_update(state)               -> False when the linear solve failed
 
-States stay on the device between calls.  Relaxation, line search and the
-time-step selectors of the reference are not part of this driver; a
-structured Schedule (control periods) is not either: `schedule` is a
+    model.apply_increment(state, scale)      the last increment again, scaled
+                                             (line search trials)
+
+States stay on the device between calls.  The reference's stabilisation is
+here: heuristic relaxation (use_relaxation, _update_relaxation :627-632),
+the back-tracking line search (use_line_search, gated like the reference
+on `convergence_issues` or always_use_stabilization, :596-614), MRST's
+per-equation oscillation and stagnation diagnosis (:427-545) and its
+relaxation types (stabilize_newton_increments :547-575).  A structured
+Schedule (control periods) is not part of this driver: `schedule` is a
 sequence of report-step lengths.
 """
 from dataclasses import dataclass, field
@@ -35,6 +42,7 @@ class StepReport:
     dt: float
     residuals: List[Dict[str, float]] = field(default_factory=list)
     cuts: int = 0
+    line_searches: int = 0
     ministeps: List[float] = field(default_factory=list)       # the ministeps a selector took, summing to dt
     linear_solves: List[tuple] = field(default_factory=list)   # (iterations, relative residual) per solve
     wall_seconds: float = 0.0
@@ -67,8 +75,16 @@ class NonLinearSolver:
     report_step_started, ministep_started, nonlinear_iteration_completed,
     linear_solve_failed, timestep_cut, step_accepted, step_rejected."""
 
+    # residual names -> the model tolerance each is judged by (MRST's per-name
+    # ResidualsConverged); unknown names count as not converged
+    _TOLERANCE_OF = {"CNV": "tol_cnv", "MB": "tol_mb", "well": "tol_well", "control": "tol_well",
+                     "extra": "tol_well"}
+
     def __init__(self, max_iterations: int = 25, max_timestep_cuts: int = 6, verbose: bool = False,
-                 observer=None, timestep_selector=None):
+                 observer=None, timestep_selector=None, use_line_search: bool = False,
+                 use_relaxation: bool = False, relaxation_min: float = 0.5, relaxation_increment: float = 0.1,
+                 line_search_max: int = 5, always_use_stabilization: bool = False,
+                 oscillation_threshold: float = 1.0):
         self.max_iterations = max_iterations
         self.max_timestep_cuts = max_timestep_cuts
         self.timestep_selector = timestep_selector
@@ -76,6 +92,126 @@ class NonLinearSolver:
         self.observer = observer
         self.observer_failures: List[dict] = []
         self._context: dict = {}
+        self.use_line_search = bool(use_line_search)
+        self.use_relaxation = bool(use_relaxation)
+        self.relaxation_min = float(relaxation_min)
+        self.relaxation_increment = float(relaxation_increment)
+        self.line_search_max = int(line_search_max)
+        self.always_use_stabilization = bool(always_use_stabilization)
+        self.oscillation_threshold = float(oscillation_threshold)
+        self.convergence_issues = False
+        # MRST's relaxation model: a fixed parameter under a named type, inert at 1
+        self.relaxation_type = "dampen"
+        self.relaxation_parameter = 1.0
+        self.previous_increment = None
+        self.stagnate_tol = 1.0e-2
+        self.identifier = ""
+        self.linesearch_convergence_names = None
+
+    # -- convergence diagnosis (MRST NonLinearSolver.m) -----------------------
+    def check_for_oscillations(self, res, index: int) -> np.ndarray:
+        """Per equation (column of res, one row per iteration): did the
+        residual change direction between iterations index-2, index-1 and
+        index?  Needs three iterations."""
+        hist = np.atleast_2d(np.asarray(res, dtype=float))
+        if index < 2:
+            return np.zeros(hist.shape[1], dtype=bool)
+        before = hist[index - 1] - hist[index - 2]
+        now = hist[index] - hist[index - 1]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = now / before
+        return np.isfinite(q) & (q < 0.0)
+
+    def check_for_stagnation(self, res, index: int) -> np.ndarray:
+        """Per equation: did the residual move by less than stagnate_tol of
+        its previous value?  Needs two iterations."""
+        hist = np.atleast_2d(np.asarray(res, dtype=float))
+        if index < 1:
+            return np.zeros(hist.shape[1], dtype=bool)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel = np.abs(hist[index] - hist[index - 1]) / hist[index - 1]
+        return np.isfinite(rel) & (rel < self.stagnate_tol)
+
+    checkForOscillations = check_for_oscillations
+    checkForStagnation = check_for_stagnation
+
+    def _converged_by_name(self, res: Dict[str, float], names, model) -> np.ndarray:
+        ok = np.zeros(len(names), dtype=bool)
+        for i, name in enumerate(names):
+            tol = getattr(model, self._TOLERANCE_OF.get(name, ""), None) if name in self._TOLERANCE_OF else None
+            if tol is not None and res.get(name) is not None:
+                ok[i] = res[name] <= tol
+        return ok
+
+    def _update_convergence_issues(self, residuals: List[Dict[str, float]], index: int, model) -> None:
+        """convergence_issues: enough of the unconverged equations oscillate
+        or stagnate (at least oscillation_threshold of them); read as the
+        next iteration's `struggling` signal (NonLinearSolver.m:402-438)."""
+        names = list(residuals[-1].keys())
+        hist = np.array([[r.get(n, np.nan) for n in names] for r in residuals], dtype=float)
+        ok = self._converged_by_name(residuals[-1], names, model)
+        if ok.all():
+            self.convergence_issues = False
+            return
+        bad = self.check_for_oscillations(hist, index) | self.check_for_stagnation(hist, index)
+        open_ = ~ok
+        self.convergence_issues = bool((bad & open_).sum() >= self.oscillation_threshold * open_.sum())
+
+    def stabilize_newton_increments(self, model, problem, dx):
+        """Relax the increment blocks dx by relaxation_type: 'dampen' scales
+        them by relaxation_parameter, 'sor' blends them with the previous
+        increment, 'none' keeps them; inert at a parameter of 1.  The
+        increment is remembered whatever the type.  Returns (dx, report)."""
+        w = float(self.relaxation_parameter)
+        kind = str(self.relaxation_type).lower()
+        if kind not in ("dampen", "sor", "none"):
+            raise ValueError(f"relaxation_type {self.relaxation_type!r} is not one of 'dampen', 'sor', 'none'")
+        if w < 1.0 and kind == "dampen":
+            dx = [w * d for d in dx]
+        elif w < 1.0 and kind == "sor" and self.previous_increment is not None:
+            dx = [w * d + (1.0 - w) * q for d, q in zip(dx, self.previous_increment)]
+        self.previous_increment = list(dx)
+        return dx, {"relaxationParameter": w, "relaxation_parameter": w}
+
+    stabilizeNewtonIncrements = stabilize_newton_increments
+
+    def get_id(self) -> str:
+        return f"{self.identifier}: " if self.identifier else ""
+
+    def get_active_names(self, names) -> np.ndarray:
+        """Which residuals the line search judges by: all, or those named
+        in linesearch_convergence_names (case-insensitive)."""
+        names = list(names)
+        if not self.linesearch_convergence_names:
+            return np.ones(len(names), dtype=bool)
+        keep = {str(n).lower() for n in self.linesearch_convergence_names}
+        return np.array([str(n).lower() in keep for n in names], dtype=bool)
+
+    getId = get_id
+    getActiveNames = get_active_names
+
+    def _update_relaxation(self, relax: float, norm: float, prev_norm: float) -> float:
+        """Damp by relaxation_increment (to relaxation_min) when the residual
+        rose, recover towards 1 when it did not."""
+        if norm > prev_norm:
+            return max(relax - self.relaxation_increment, self.relaxation_min)
+        return min(relax + self.relaxation_increment, 1.0)
+
+    def _line_search(self, model, state0, base, relax: float, dt: float, norm0: float):
+        """Halve the step along the last increment from `base` until the
+        residual norm is no larger than norm0 (at most line_search_max
+        trials).  Returns (state, trials)."""
+        alpha = relax
+        trial = base.copy()
+        model.apply_increment(trial, alpha)
+        for i in range(self.line_search_max):
+            _, res = model.linearize(state0, trial, dt)
+            if max(res.values()) <= norm0:
+                return trial, i
+            alpha *= 0.5
+            trial = base.copy()
+            model.apply_increment(trial, alpha)
+        return trial, self.line_search_max
 
     def _observe(self, event: str, **payload):
         if self.observer is None:
@@ -91,6 +227,9 @@ class NonLinearSolver:
         self._observe("ministep_started", dt=float(dt))
         started = perf_counter()
         st = state
+        relax, prev_norm = 1.0, np.inf
+        self.convergence_issues = False
+        stabilising = self.use_relaxation or self.use_line_search or self.always_use_stabilization
         model.begin_ministep(state0)
         for it in range(self.max_iterations):
             t0 = perf_counter()
@@ -98,13 +237,22 @@ class NonLinearSolver:
             report.assembly_seconds += perf_counter() - t0
             report.residuals.append(res)
             report.iterations = it
+            norm = max(res.values()) if res else 0.0
             self._observe("nonlinear_iteration_completed", iteration=int(it), dt=float(dt),
-                          converged=bool(converged), residuals=dict(res))
+                          converged=bool(converged), residuals=dict(res), relaxation=float(relax))
             if converged:
                 report.converged = True
                 report.wall_seconds = perf_counter() - started
+                self.convergence_issues = False
                 return st, report
-            ok = model.newton_update(st)
+            # the previous iteration's diagnosis decides this one's line search
+            struggling = self.always_use_stabilization or self.convergence_issues
+            if self.use_relaxation:
+                relax = self._update_relaxation(relax, norm, prev_norm)
+            prev_norm = norm
+            searching = self.use_line_search and struggling
+            base = st.copy() if searching else None
+            ok = model.newton_update(st, relax) if relax != 1.0 else model.newton_update(st)
             report.linear_solves.append(model.last_linear_solve)
             if not ok:
                 # an iterative solve that fails is a time-step failure: the
@@ -114,12 +262,18 @@ class NonLinearSolver:
                               linear_iterations=int(model.last_linear_solve[0]))
                 report.wall_seconds = perf_counter() - started
                 return st, report
+            if searching:
+                st, trials = self._line_search(model, state0, base, relax, dt, norm)
+                report.line_searches += trials
+            if stabilising:
+                self._update_convergence_issues(report.residuals, it, model)
+        self.convergence_issues = False
         t0 = perf_counter()
         converged, res = model.linearize(state0, st, dt)
         report.assembly_seconds += perf_counter() - t0
         report.residuals.append(res)
         self._observe("nonlinear_iteration_completed", iteration=int(self.max_iterations), dt=float(dt),
-                      converged=bool(converged), residuals=dict(res))
+                      converged=bool(converged), residuals=dict(res), relaxation=float(relax))
         report.converged = bool(converged)
         report.iterations = self.max_iterations
         report.wall_seconds = perf_counter() - started

@@ -381,7 +381,7 @@ class TwoPhaseOilWaterModelAD:
         self.last_cnv_mb = (cnv, mb)
         return ok, out7[6] / nc
 
-    def solve_and_update(self, st: DeviceOWState, pmean: float) -> bool:
+    def solve_and_update(self, st: DeviceOWState, pmean: float, scale: float = 1.0) -> bool:
         info = _lib.SolveInfo()
         opts = self.linear_solver.options()
         t0 = time.perf_counter()
@@ -395,10 +395,14 @@ class TwoPhaseOilWaterModelAD:
         self.stats["linear"] += int(info.iterations)
         if rc != 0:
             return False
-        P = _lib.ptr
-        _lib.check(self.lib.pmx_ow_update(self.h, P(self._dx), 1.0, self.ds_max, self.dp_max_rel, P(st.p),
-                                          P(st.sw), P(st.qws), P(st.qos), P(st.bhp)), "pmx_ow_update")
+        self.apply_increment(st, scale)
         return True
+
+    def apply_increment(self, st: DeviceOWState, scale: float = 1.0):
+        """st += scale * (the last Newton increment), with the update limits."""
+        P = _lib.ptr
+        _lib.check(self.lib.pmx_ow_update(self.h, P(self._dx), float(scale), self.ds_max, self.dp_max_rel, P(st.p),
+                                          P(st.sw), P(st.qws), P(st.qos), P(st.bhp)), "pmx_ow_update")
 
     # -- wells: hydrostatic connection pressure drops, once per step ----------
     def compute_cdp_per_well(self, st0: DeviceOWState) -> Optional[torch.Tensor]:
@@ -444,9 +448,9 @@ class TwoPhaseOilWaterModelAD:
         cnv, mb = self.last_cnv_mb
         return ok, {"CNV": float(cnv), "MB": float(mb)}
 
-    def newton_update(self, st: DeviceOWState) -> bool:
+    def newton_update(self, st: DeviceOWState, scale: float = 1.0) -> bool:
         self.stats["newton"] += 1
-        return self.solve_and_update(st, self._ms_pmean)
+        return self.solve_and_update(st, self._ms_pmean, scale)
 
     @property
     def last_linear_solve(self):

@@ -239,6 +239,33 @@ def test_nonlinear_solver_drives_the_oil_water_model():
         np.testing.assert_allclose(a.sW, b.sW, rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("options", [dict(use_relaxation=True),
+                                     dict(use_relaxation=True, use_line_search=True, always_use_stabilization=True)])
+def test_stabilised_newton_matches_the_oracle(options):
+    """Relaxation and the back-tracking line search (with the stabilisation
+    forced on) take the oracle's stabilised Newton path: the same Newton
+    iterations, cuts and line-search trials per report step, states to 1e-8
+    (nonlinear_solver.py:239-381, 596-648)."""
+    from pymrst_amd.nonlinear import NonLinearSolver
+    from pymrst_amd.ow import CPRSolver
+    from oracle.nonlinear import StabilisedNewton, simulate_stabilised
+    inp = small_gravity_inputs()
+    dts = np.full(3, 30.0 * og.DAY)
+    mo, so = build_oracle(inp)
+    ref_states, ref_its, ref_cuts, ref_ls = simulate_stabilised(mo, so, dts, StabilisedNewton(**options))
+    md, sd = build_device(inp, CPRSolver(rtol=1e-12))
+    states, rep = NonLinearSolver(**options).simulate_schedule(md, sd, dts)
+    assert rep.converged
+    assert [s.iterations for s in rep.steps] == ref_its.tolist()
+    assert [s.cuts for s in rep.steps] == ref_cuts.tolist()
+    assert [s.line_searches for s in rep.steps] == ref_ls.tolist()
+    if options.get("use_line_search"):
+        assert ref_ls.sum() > 0          # the line search did back off
+    for a, b in zip(states, ref_states):
+        np.testing.assert_allclose(a.pressure, b.pressure, rtol=1e-8)
+        np.testing.assert_allclose(a.sW, b.sW, rtol=0, atol=1e-8)
+
+
 @pytest.mark.parametrize("prec", [64, 32])
 def test_cpr_solve_on_a_level_large_enough_for_the_ell_smoother(prec):
     """A 66x64x64 field case (270,336 pressure rows: the AMG's first level
