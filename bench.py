@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # calibrated against a 512 MiB copy (fetch factor 2.0, write factor 1.0):
 # 404.3 MB read + 17.95 MB written.
 MEASURED_SPMV_TRAFFIC = {(60, 220, 85): 422.2e6}
-# k_line_gs_tw<2,float,16> per launch (one colour), FETCH_SIZE x 2 + WRITE_SIZE: 40.35 + 30.47 MB
-MEASURED_LINE_GS_TRAFFIC = {(60, 220, 85): 70.82e6}
+# k_line_gs_tw<2,float,8> per launch (one colour)
+MEASURED_LINE_GS_TRAFFIC = {}
 
 
 def spmv_bytes(nnzb: int, nc: int) -> float:
@@ -544,8 +544,9 @@ def main():
             per_launch = lg["bytes"] / lg["launches"]
             achieved = per_launch / avg_s / 1e9
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": MEASURED_LINE_GS_TRAFFIC.get(dims),
-                    "kernel": "k_line_gs_tw<2,float,16> (twisted block-tridiagonal line solves of one colour, "
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": MEASURED_LINE_GS_TRAFFIC.get(dims),
+                    "kernel": "k_line_gs_tw<2,float,8> (twisted block-tridiagonal line solves of one colour, "
                               "the CPR local stage; the kernel with the most GPU time)",
                     "bytes_per_launch": round(per_launch), "avg_launch_us": round(avg_s * 1e6, 2),
                     "launches": lg["launches"]}

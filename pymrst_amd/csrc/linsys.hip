@@ -45,6 +45,17 @@ struct PrecIO {
   double* z;
 };
 
+// The pointers read from PrecIO are generic as far as the compiler knows, so
+// their stores would be FLAT; a pending FLAT store makes every later wait on
+// a load a wait for all outstanding memory operations, which serialises the
+// prefetch of a walk.  These casts tell the compiler they are global memory.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem<T>* global_ptr(T* p) {
+  return (gmem<T>*)p;
+}
+
 __global__ void k_set_io(PrecIO* io, const double* r, double* z) {
   io->r = r;
   io->z = z;
@@ -710,7 +721,7 @@ __global__ void k_weight_restrict(int nc, const double* __restrict__ w, const Pr
                                   T* __restrict__ rp) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
-  const double* __restrict__ r = io->r;
+  const gmem<double>* __restrict__ r = global_ptr(io->r);
   double s = 0.0;
 #pragma unroll
   for (int a = 0; a < B; ++a) s += w[a] * r[(size_t)c * B + a];
@@ -723,8 +734,8 @@ __global__ __launch_bounds__(256) void k_cpr_stage2(int nc, const int* __restric
                                                     const int* __restrict__ col, const T* __restrict__ Jp,
                                                     const T* __restrict__ xp, const PrecIO* __restrict__ io,
                                                     const T* __restrict__ inv, const double* __restrict__ pc) {
-  const double* __restrict__ r = io->r;
-  double* __restrict__ z = io->z;
+  const gmem<double>* __restrict__ r = global_ptr(io->r);
+  gmem<double>* __restrict__ z = global_ptr(io->z);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = t / LPR, lane = t % LPR;
   double acc[B];
@@ -762,8 +773,8 @@ __global__ __launch_bounds__(64) void k_cpr_wells(int nw, const int* __restrict_
                                                   const T* __restrict__ xp, const PrecIO* __restrict__ io,
                                                   size_t ncB, double* __restrict__ hp,
                                                   const double* __restrict__ pc) {
-  const double* __restrict__ r = io->r;
-  double* __restrict__ z = io->z;
+  const gmem<double>* __restrict__ r = global_ptr(io->r);
+  gmem<double>* __restrict__ z = global_ptr(io->z);
   const int w = blockIdx.x, lane = threadIdx.x;
   double ay[M], ah[X];
 #pragma unroll
@@ -810,7 +821,7 @@ __global__ void k_cpr_mixfix(int npc, const int* __restrict__ pc_cell, const int
                              const T* __restrict__ inv, const PrecIO* __restrict__ io) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= npc) return;
-  double* __restrict__ z = io->z;
+  gmem<double>* __restrict__ z = global_ptr(io->z);
   const int c = pc_cell[u];
   double acc[B];
 #pragma unroll
@@ -871,7 +882,7 @@ __global__ __launch_bounds__(256) void k_line_r1(int nc, const int* __restrict__
                                                  const int* __restrict__ perf_well, const double* __restrict__ Um,
                                                  const double* __restrict__ hp, const int* __restrict__ pos,
                                                  T* __restrict__ r1l, T* __restrict__ xpl) {
-  const double* __restrict__ r = io->r;
+  const gmem<double>* __restrict__ r = global_ptr(io->r);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = t / LPR, lane = t % LPR;
   double acc[B];
@@ -932,7 +943,7 @@ __global__ __launch_bounds__(256) void k_line_r1_sell(int nc, const int* __restr
                                                       const double* __restrict__ Um, const double* __restrict__ hp,
                                                       const int* __restrict__ pos, T* __restrict__ r1l,
                                                       T* __restrict__ xpl) {
-  const double* __restrict__ r = io->r;
+  const gmem<double>* __restrict__ r = global_ptr(io->r);
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= nc) return;
   const int sl = row >> 6;
@@ -970,12 +981,14 @@ __global__ __launch_bounds__(256) void k_line_r1_sell(int nc, const int* __restr
 // Right-hand sides of the lines of colour `color` (one thread per line
 // position): r2 = r1 - sum_h Aoff_h y_nbr_h over the couplings off the line;
 // in the first sweep only neighbours of earlier colours have a value yet.
+// The line results y are kept by table position (ypos), so off_nbr holds the
+// neighbours' table positions.
 template <int B, class T>
 __global__ __launch_bounds__(256) void k_line_rhs(int nl, int lc0, int nlc, int Lc, int color, int first, int noff,
                                                   size_t npos, const int* __restrict__ len,
                                                   const int* __restrict__ off_nbr,
                                                   const unsigned char* __restrict__ off_col,
-                                                  const T* __restrict__ aoff, const T* __restrict__ ycell,
+                                                  const T* __restrict__ aoff, const T* __restrict__ ypos,
                                                   const T* __restrict__ r1l, T* __restrict__ r2l) {
   constexpr int BB = B * B;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -992,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_line_rhs(int nl, int lc0, int nlc, int 
     if (j < 0 || (first && off_col[e] >= color)) continue;
     double yj[B];
 #pragma unroll
-    for (int b = 0; b < B; ++b) yj[b] = (double)ycell[(size_t)j * B + b];
+    for (int b = 0; b < B; ++b) yj[b] = (double)ypos[(size_t)j * B + b];
 #pragma unroll
     for (int a = 0; a < B; ++a)
 #pragma unroll
@@ -1018,12 +1031,14 @@ __global__ void k_gather_off(size_t ne, const int* __restrict__ off_q, const dou
 // without a data-dependent index), and the scheduling barriers keep the
 // compiler from sinking those loads next to their use.  One lane per whole
 // line (a single LU walk down and back) took 29.9 us per colour at 60x220x85
-// against 22.4 us for the twisted form; premultiplying D'^{-1} U to move fewer
-// bytes was 7% slower still: the walks are bound by their dependent chains
-// and the bytes in flight, not by bandwidth.
+// against 22.4 us for the twisted form.  With one wave per line group the
+// walks are bound by memory latency per step, not by their arithmetic (the
+// same recurrence fed from registers runs in under 1 us) nor by the
+// prefetch depth (8 and 16 run alike; 8 by a few percent): what shortens a
+// step is fewer memory operations in it.
 template <int B, class T>
 constexpr int line_depth() {
-  return B * B * (int)sizeof(T) <= 16 ? 16 : (B * B * (int)sizeof(T) <= 36 ? 8 : 4);
+  return B * B * (int)sizeof(T) <= 36 ? 8 : 4;
 }
 
 // Twisted ("burn at both ends") block factorization of each line: the top
@@ -1136,16 +1151,20 @@ __global__ void k_line_factor_tw(int nl, const int* __restrict__ len, const int*
 // line meet at its middle cell through a lane exchange, and both walk back
 // out.  Every load is at a position inside its line (steps past a lane's
 // walk are clamped and their results dropped), so lines of a wave may differ
-// in length.
-template <int B, class T, int D>
+// in length.  The results go to ypos by table position (consecutive lanes,
+// consecutive addresses), where the next colour's right-hand sides read
+// them; only the last sweep (Z) also writes the output z = x + pc xp by cell,
+// since each sweep overwrites every line cell's z and nothing reads z between
+// sweeps.
+template <int B, class T, int D, bool Z>
 __global__ __launch_bounds__(64) void k_line_gs_tw(int nl, int lc0, const int* __restrict__ len,
                                                    const int* __restrict__ cell, const T* __restrict__ inv,
                                                    const T* __restrict__ Lo, const T* __restrict__ Up,
                                                    T* __restrict__ r2l, const T* __restrict__ xpl,
-                                                   const PrecIO* __restrict__ io, T* __restrict__ ycell,
+                                                   const PrecIO* __restrict__ io, T* __restrict__ ypos,
                                                    const double* __restrict__ pcw) {
   constexpr int BB = B * B;
-  double* __restrict__ z = io->z;
+  gmem<double>* __restrict__ z = global_ptr(io->z);
   double pc[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) pc[a] = pcw[a];
@@ -1236,32 +1255,39 @@ __global__ __launch_bounds__(64) void k_line_gs_tw(int nl, int lc0, const int* _
       for (int q = 0; q < B; ++q) v += (double)inv[at * BB + a * B + q] * rhs[q];
       xm[a] = v;
     }
-    const int c = cell[at];
-    if (h == 0 && c >= 0) {
-      const double xp = (double)xpl[at];
+    if (h == 0) {
+      if (ypos != nullptr)
 #pragma unroll
-      for (int a = 0; a < B; ++a) {
-        z[(size_t)c * B + a] = xm[a] + pc[a] * xp;
-        if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xm[a];
+        for (int a = 0; a < B; ++a) ypos[at * B + a] = (T)xm[a];
+      if (Z) {
+        const int c = cell[at];
+        const double xp = (double)xpl[at];
+        if (c >= 0)
+#pragma unroll
+          for (int a = 0; a < B; ++a) z[(size_t)c * B + a] = xm[a] + pc[a] * xp;
       }
     }
   }
+  // back out: the forward results y come from r2l, the pressure correction
+  // and the cell index only in the last sweep
   T bi[D][BB], bu[D][BB], by[D][B], bx[D];
   int bc[D];
-  auto bload = [&](int j, T (&vi)[BB], T (&vu)[BB], T (&vy)[B], T& vx, int& c) {
+  auto bload = [&](int j, int u) {
     const size_t at = (size_t)pos(j) * nl + l;
 #pragma unroll
     for (int q = 0; q < BB; ++q) {
-      vi[q] = inv[at * BB + q];
-      vu[q] = Up[at * BB + q];
+      bi[u][q] = inv[at * BB + q];
+      bu[u][q] = Up[at * BB + q];
     }
 #pragma unroll
-    for (int a = 0; a < B; ++a) vy[a] = r2l[at * B + a];
-    vx = xpl[at];
-    c = cell[at];
+    for (int a = 0; a < B; ++a) by[u][a] = r2l[at * B + a];
+    if (Z) {
+      bx[u] = xpl[at];
+      bc[u] = cell[at];
+    }
   };
 #pragma unroll
-  for (int u = 0; u < D; ++u) bload(Jmax - 1 - u, bi[u], bu[u], by[u], bx[u], bc[u]);
+  for (int u = 0; u < D; ++u) bload(Jmax - 1 - u, u);
   double xn[B];
 #pragma unroll
   for (int a = 0; a < B; ++a) xn[a] = xm[a];
@@ -1287,16 +1313,18 @@ __global__ __launch_bounds__(64) void k_line_gs_tw(int nl, int lc0, const int* _
       if (j < J) {
 #pragma unroll
         for (int a = 0; a < B; ++a) xn[a] = xv[a];
-        const int c = bc[u];
-        if (c >= 0) {
+        if (ypos != nullptr) {
+          const size_t at = (size_t)pos(j) * nl + l;
 #pragma unroll
-          for (int a = 0; a < B; ++a) {
-            z[(size_t)c * B + a] = xv[a] + pc[a] * (double)bx[u];
-            if (ycell != nullptr) ycell[(size_t)c * B + a] = (T)xv[a];
-          }
+          for (int a = 0; a < B; ++a) ypos[at * B + a] = (T)xv[a];
+        }
+        if (Z && bc[u] >= 0) {
+          const size_t c = (size_t)bc[u];
+#pragma unroll
+          for (int a = 0; a < B; ++a) z[c * B + a] = xv[a] + pc[a] * (double)bx[u];
         }
       }
-      bload(j - D, bi[u], bu[u], by[u], bx[u], bc[u]);
+      bload(j - D, u);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
@@ -1420,7 +1448,7 @@ __global__ void k_ilu_upper_level(int nr, const int* __restrict__ rows, const in
     for (int b = 0; b < B; ++b) s += (double)Dinv[(size_t)i * BB + a * B + b] * v[b];
     x[a] = s;
   }
-  double* __restrict__ z = io->z;
+  gmem<double>* __restrict__ z = global_ptr(io->z);
   const double xpi = (double)xp[i];
 #pragma unroll
   for (int a = 0; a < B; ++a) {
@@ -2043,7 +2071,7 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::lf_up(*C).alloc(m);
     V::r2(*C).alloc(pos * B);
     V::xpl(*C).alloc(pos);
-    V::yc(*C).alloc((size_t)S.nc * B);
+    V::yc(*C).alloc(pos * B);
     V::r1l(*C).alloc(pos * B);
     V::r1l(*C).zero(s);
     const size_t ne = (size_t)C->noff * pos;
@@ -2359,24 +2387,27 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
                            nlc, Lc, c, sw == 0 ? 1 : 0, C->noff, npos, C->ln_len.p, C->off_nbr.p, C->off_col.p,
                            V::aoff(*C).p, V::yc(*C).p, V::r1l(*C).p, V::r2(*C).p);
         launch_check();
+        const bool zs = sw + 1 == sweeps;               // the last sweep writes the output
         hipEvent_t e0 = nullptr, e1 = nullptr;
         const bool timed = !C->capturing && C->lgs_t.pair(&e0, &e1);
+        auto kern = zs ? k_line_gs_tw<B, T, D, true> : k_line_gs_tw<B, T, D, false>;
         if (timed)
-          hipExtLaunchKernelGGL((k_line_gs_tw<B, T, D>), dim3(nlc / 32), dim3(64), 0, s, e0, e1, 0,
-                                C->nlines, lc0, C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p,
-                                V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p,
-                                C->pcdev);
+          hipExtLaunchKernelGGL(kern, dim3(nlc / 32), dim3(64), 0, s, e0, e1, 0, C->nlines, lc0, C->ln_len.p,
+                                C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p,
+                                V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
         else
-          hipLaunchKernelGGL((k_line_gs_tw<B, T, D>), dim3(nlc / 32), dim3(64), 0, s, C->nlines, lc0,
-                             C->ln_len.p, C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p,
-                             V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
+          hipLaunchKernelGGL(kern, dim3(nlc / 32), dim3(64), 0, s, C->nlines, lc0, C->ln_len.p, C->ln_cell.p,
+                             V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::r2(*C).p, V::xpl(*C).p, C->io,
+                             last ? nullptr : V::yc(*C).p, C->pcdev);
         launch_check();
         if (timed) {
           // per line cell: forward D'^-1, L, r read and y written; backward
-          // D'^-1, U, y, the pressure xp and the cell index read, z (FP64)
-          // written, and y_cell unless this is the last launch
+          // D'^-1, U, y read, and the result written by table position
+          // unless this is the last launch; in the last sweep also the
+          // pressure xp and the cell index read and z (FP64) written
           const double sv = sizeof(T);
-          const double per_cell = 4.0 * B * B * sv + 3.0 * B * sv + sv + 4.0 + 8.0 * B + (last ? 0.0 : B * sv);
+          const double per_cell = 4.0 * B * B * sv + 3.0 * B * sv + (last ? 0.0 : B * sv) +
+                                  (zs ? sv + 4.0 + 8.0 * B : 0.0);
           C->lgs_bytes += per_cell * (double)C->color_cells[c];
         }
       }

@@ -100,8 +100,8 @@ constexpr int LINE_PAD = 16;
 constexpr int LINE_GROUP = 64;
 // The couplings of each line cell to cells off its line are listed in noff
 // position-major planes (plane h, entry h*maxlen*nlines + k*nlines + l):
-// the BCSR entry, the neighbour cell and the neighbour's colour (-1 / 255
-// where a cell has fewer).  pos[c] is the table position of cell c.
+// the BCSR entry, the neighbour's table position and the neighbour's colour
+// (-1 / 255 where a cell has fewer).  pos[c] is the table position of cell c.
 struct HostLines {
   int nlines = 0, maxlen = 0, longest = 0, ncolor = 0, noff = 0;
   std::vector<int> len, cell, lo, up, color_ptr, pos;

@@ -379,7 +379,7 @@ static void line_tables(const HostPattern& P, std::vector<std::vector<int>>& lin
       const int j = P.col[q];
       if (lof[j] == lof[c]) continue;
       L.off_q[h * npos + at] = q;
-      L.off_nbr[h * npos + at] = j;
+      L.off_nbr[h * npos + at] = L.pos[j];
       L.off_col[h * npos + at] = L.cell_color[j];
       ++h;
     }
