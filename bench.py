@@ -53,7 +53,7 @@ OW_CPR = {"local_sweeps": 5, "jacobi_omega": 1.0}
 BO_CPR = {"local_sweeps": 3}
 
 
-def run_form(case, precision, steps, warmup, rtol, local_stage=None):
+def run_form(case, precision, steps, warmup, rtol, local_stage=None, line_segments=False):
     """Time `steps` report steps of the schedule: passes over it from the
     initial state, the last one cut short, so any step count is exact."""
     import torch
@@ -62,6 +62,8 @@ def run_form(case, precision, steps, warmup, rtol, local_stage=None):
     cpr = dict(OW_CPR)
     if local_stage is not None:
         cpr["local_stage"] = local_stage
+    if line_segments:
+        cpr["line_segments"] = True
     # no per-solve log lines inside the timed region (the observer reports each report step)
     solver = CPRSolver(rtol=rtol, precision=precision, verbose=0, **cpr)
     t_setup = time.perf_counter()
@@ -469,6 +471,8 @@ def main():
     ap.add_argument("--ensemble-loop", type=int, default=16, help="members timed in the looped plain form")
     ap.add_argument("--local-stage", choices=("line", "ilu", "block_jacobi"), default=None,
                     help="CPR local stage of the oil-water legs (default: line Gauss-Seidel)")
+    ap.add_argument("--line-segments", action="store_true",
+                    help="solve lines of 32+ cells by segments with spikes (partitioned) instead of one walk")
     ap.add_argument("--no-incomp", action="store_true")
     ap.add_argument("--incomp-only", action="store_true")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
@@ -504,7 +508,7 @@ def main():
     forms = {"both": (64, 32), "mixed": (32,), "fp64": (64,)}[args.form]
     res = {}
     for prec in forms:
-        res[prec] = run_form(case, prec, steps, args.warmup, args.rtol, args.local_stage)
+        res[prec] = run_form(case, prec, steps, args.warmup, args.rtol, args.local_stage, args.line_segments)
         r = res[prec]
         print(f"[bench] FP{prec} preconditioner: {int(r['its'].sum())} Newton its, {r['lin_its']} GMRES its, "
               f"{r['wall']:.3f} s", file=sys.stderr, flush=True)
@@ -546,8 +550,10 @@ def main():
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": MEASURED_LINE_GS_TRAFFIC.get(dims),
-                    "kernel": "k_line_gs_tw<2,float,8> (twisted block-tridiagonal line solves of one colour, "
-                              "the CPR local stage; the kernel with the most GPU time)",
+                    "kernel": ("k_line_gs_seg<2,float,8,4> (partitioned" if args.line_segments else
+                               "k_line_gs_tw<2,float,8> (twisted") +
+                              " block-tridiagonal line solves of one colour, the CPR local stage; the kernel with "
+                              "the most GPU time)",
                     "bytes_per_launch": round(per_launch), "avg_launch_us": round(avg_s * 1e6, 2),
                     "launches": lg["launches"]}
     if roof is None:
@@ -566,7 +572,8 @@ def main():
                                    f"the {case.dts.size}-step schedule from the initial state), CPR-FGMRES rtol "
                                    f"{args.rtol}",
                        "cells": nc, "report_steps": steps,
-                       "cpr": dict(OW_CPR, **({"local_stage": args.local_stage} if args.local_stage else {}))},
+                       "cpr": dict(OW_CPR, **({"local_stage": args.local_stage} if args.local_stage else {}),
+                                   **({"line_segments": True} if args.line_segments else {}))},
             "roofline": roof, "roofline_spmv": roof_spmv, "cpu_baseline": cpu}
     line.update(extra)
     line["stream_copy"] = stream_copy()

@@ -45,6 +45,8 @@ typedef struct pmx_solver_options {
   int32_t local_sweeps;     /* Gauss-Seidel sweeps over the line colours */
   int32_t use_graphs;       /* 1: replay the preconditioner as a captured HIP graph */
   int32_t coarse_block_rows; /* AMG levels up to this many rows run in one workgroup */
+  int32_t line_segments;    /* 1: lines of 32+ cells solved by segments (spikes and a Schur system on
+                               separator cells); 0: each line by one twisted walk (default) */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {

@@ -34,6 +34,7 @@ class SolverOptions(ctypes.Structure):
         ("local_sweeps", ctypes.c_int32),
         ("use_graphs", ctypes.c_int32),
         ("coarse_block_rows", ctypes.c_int32),
+        ("line_segments", ctypes.c_int32),
     ]
 
 

@@ -81,6 +81,7 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.local_sweeps = o->local_sweeps;
   r.use_graphs = o->use_graphs;
   r.coarse_block_rows = o->coarse_block_rows;
+  r.line_segments = o->line_segments;
   return r;
 }
 
@@ -123,6 +124,7 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->local_sweeps = d.local_sweeps;
   o->use_graphs = d.use_graphs;
   o->coarse_block_rows = d.coarse_block_rows;
+  o->line_segments = d.line_segments;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {

@@ -80,6 +80,7 @@ struct LinearSolverOptions {
   int local_sweeps = 4;      // Gauss-Seidel sweeps of the line local stage
   int use_graphs = 1;        // replay the preconditioner application as a captured HIP graph
   int coarse_block_rows = 0;     // AMG levels up to this size run in one workgroup (the coarsest always does)
+  int line_segments = 0;     // lines of 32+ cells solved by segments with spikes (measured slower: off)
 };
 
 struct SolveInfo {

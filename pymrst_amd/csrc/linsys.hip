@@ -1050,14 +1050,15 @@ constexpr int line_depth() {
 // below), so both halves are solved by the same walk: forward from the line's
 // end to the middle, then back out.
 template <int B, class T>
-__global__ void k_line_factor_tw(int nl, const int* __restrict__ len, const int* __restrict__ cell,
-                                 const int* __restrict__ lo, const int* __restrict__ up,
-                                 const int* __restrict__ diag, const double* __restrict__ val,
-                                 T* __restrict__ inv, T* __restrict__ Lo, T* __restrict__ Up) {
+__global__ void k_line_factor_tw(int nl, const int* __restrict__ len, const int* __restrict__ nseg,
+                                 const int* __restrict__ cell, const int* __restrict__ lo,
+                                 const int* __restrict__ up, const int* __restrict__ diag,
+                                 const double* __restrict__ val, T* __restrict__ inv, T* __restrict__ Lo,
+                                 T* __restrict__ Up) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= nl) return;
   const int L = len[l];
-  if (L == 0) return;
+  if (L == 0 || (nseg != nullptr && nseg[l] > 0)) return;  // partitioned lines: k_line_factor_seg
   const int m = L / 2;
   double zero[B][B];
 #pragma unroll
@@ -1327,6 +1328,547 @@ __global__ __launch_bounds__(64) void k_line_gs_tw(int nl, int lc0, const int* _
       bload(j - D, u);
       __builtin_amdgcn_sched_barrier(0);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned line solves.  A line of L >= 32 cells is cut at ns - 1
+// separator cells into ns segments of 15-31 cells (ns = 2, 4 or 8); each
+// segment is factored and walked like a line of its own (the twisted form,
+// two lanes per segment), which solves A_ii g_i = r_i with the separators
+// held at zero.  The spikes W_L = A_ii^{-1} E_first A(first, s_left) and
+// W_R = A_ii^{-1} E_last A(last, s_right) give every segment cell's response
+// to its separators, so the separators satisfy the block-tridiagonal Schur
+// system
+//   (A_ss - L_s W_R,k[last] - U_s W_L,k+1[first]) x_s - L_s W_L,k[last] x_s-
+//     - U_s W_R,k+1[first] x_s+ = r_s - L_s g_k[last] - U_s g_k+1[first],
+// (L_s, U_s the separator's couplings to the cells before and after it),
+// factored once per setup by block Thomas, and each segment cell is then
+// x = g - W_L x_s_left - W_R x_s_right.  Exact in exact arithmetic, with
+// walks of a quarter (ns = 4) of the twisted walk's length and ns times as
+// many lanes per line.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int line_nseg(int L) {
+  const int n = L / 16;
+  return n >= 8 ? 8 : (n >= 4 ? 4 : (n >= 2 ? 2 : 1));
+}
+// separator k (k < ns - 1) of a line of L cells cut into ns segments
+__host__ __device__ inline int line_sep(int L, int ns, int k) { return (k + 1) * (L + 1) / ns - 1; }
+// first and last cell of segment k
+__host__ __device__ inline void line_segment(int L, int ns, int k, int& a, int& b) {
+  a = k == 0 ? 0 : line_sep(L, ns, k - 1) + 1;
+  b = k == ns - 1 ? L - 1 : line_sep(L, ns, k) - 1;
+}
+
+template <int B>
+struct Blk {
+  double v[B][B];
+  __device__ void zero() {
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) v[i][j] = 0.0;
+  }
+  template <class T>
+  __device__ void load(const T* p) {
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) v[i][j] = (double)p[i * B + j];
+  }
+  template <class T>
+  __device__ void store(T* p) const {
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) p[i * B + j] = (T)v[i][j];
+  }
+};
+template <int B>
+__device__ __forceinline__ Blk<B> bmul(const Blk<B>& x, const Blk<B>& y) {
+  Blk<B> o;
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) s += x.v[i][k] * y.v[k][j];
+      o.v[i][j] = s;
+    }
+  return o;
+}
+template <int B>
+__device__ __forceinline__ Blk<B> bsub(const Blk<B>& x, const Blk<B>& y) {
+  Blk<B> o;
+#pragma unroll
+  for (int i = 0; i < B; ++i)
+#pragma unroll
+    for (int j = 0; j < B; ++j) o.v[i][j] = x.v[i][j] - y.v[i][j];
+  return o;
+}
+template <int B>
+__device__ __forceinline__ Blk<B> binv(const Blk<B>& x) {
+  Blk<B> a = x, o;
+  small_inverse<B>(a.v, o.v);
+  return o;
+}
+
+// Twisted factors of every segment of the lines with nseg[l] > 0 (one thread
+// per segment), with the coupling back towards the middle premultiplied by
+// the pivot's inverse off the middle (P = D'^{-1} U: the backward step is
+// g_k = y_k - P_k g_k+1), then the segment's spikes, solved with the stored
+// (rounded) factors as the walks will apply them; a segment at a line's end
+// has no spike on that side (zero).
+template <int B, class T>
+__global__ void k_line_factor_seg(int nl, const int* __restrict__ len, const int* __restrict__ nseg,
+                                  const int* __restrict__ cell, const int* __restrict__ lo,
+                                  const int* __restrict__ up, const int* __restrict__ diag,
+                                  const double* __restrict__ val, T* __restrict__ inv, T* __restrict__ Lo,
+                                  T* __restrict__ Up, T* __restrict__ WL, T* __restrict__ WR) {
+  constexpr int BB = B * B;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = t / 8, sg = t % 8;
+  if (l >= nl) return;
+  const int L = len[l], ns = nseg[l];
+  if (L == 0 || ns == 0 || sg >= ns) return;
+  int a, b;
+  line_segment(L, ns, sg, a, b);
+  const int m = a + (b - a + 1) / 2;
+  auto AT = [&](int k) -> size_t { return (size_t)k * nl + l; };
+  auto vblk = [&](int q) { Blk<B> x; x.load(val + (size_t)q * BB); return x; };
+  Blk<B> zero;
+  zero.zero();
+  // factors: top half [a, m) downwards, bottom half (m, b] upwards
+  Blk<B> Pt, Ut, Pb, Lb;
+  for (int k = a; k < m; ++k) {
+    const size_t at = AT(k);
+    Blk<B> Dk = vblk(diag[cell[at]]), A = zero;
+    if (k > a) {
+      A = vblk(lo[at]);
+      Dk = bsub(Dk, bmul(bmul(A, Pt), Ut));
+    }
+    A.store(Lo + at * BB);
+    Pt = binv(Dk);
+    Pt.store(inv + at * BB);
+    Ut = vblk(up[at]);
+    bmul(Pt, Ut).store(Up + at * BB);
+  }
+  for (int k = b; k > m; --k) {
+    const size_t at = AT(k);
+    Blk<B> Dk = vblk(diag[cell[at]]), A = zero;
+    if (k < b) {
+      A = vblk(up[at]);
+      Dk = bsub(Dk, bmul(bmul(A, Pb), Lb));
+    }
+    A.store(Lo + at * BB);
+    Pb = binv(Dk);
+    Pb.store(inv + at * BB);
+    Lb = vblk(lo[at]);
+    bmul(Pb, Lb).store(Up + at * BB);
+  }
+  {
+    const size_t at = AT(m);
+    Blk<B> Dk = vblk(diag[cell[at]]), A = zero, C = zero;
+    if (m > a) {
+      A = vblk(lo[at]);
+      Dk = bsub(Dk, bmul(bmul(A, Pt), Ut));
+    }
+    if (m < b) {
+      C = vblk(up[at]);
+      Dk = bsub(Dk, bmul(bmul(C, Pb), Lb));
+    }
+    A.store(Lo + at * BB);
+    C.store(Up + at * BB);
+    binv(Dk).store(inv + at * BB);
+  }
+  // spikes: A_seg W = E_a A(a, a-1) (left) or E_b A(b, b+1) (right); the
+  // forward results are kept in W until the backward walk overwrites them
+  auto spike = [&](T* W, bool left) {
+    const Blk<B> R = left ? vblk(lo[AT(a)]) : vblk(up[AT(b)]);
+    Blk<B> yp = zero, fi, fl;
+    for (int k = a; k < m; ++k) {
+      const size_t at = AT(k);
+      fi.load(inv + at * BB);
+      fl.load(Lo + at * BB);
+      const Blk<B> rk = (left && k == a) ? R : zero;
+      yp = bmul(fi, bsub(rk, bmul(fl, yp)));
+      yp.store(W + at * BB);
+    }
+    const Blk<B> ytop = yp;
+    yp = zero;
+    for (int k = b; k > m; --k) {
+      const size_t at = AT(k);
+      fi.load(inv + at * BB);
+      fl.load(Lo + at * BB);
+      const Blk<B> rk = (!left && k == b) ? R : zero;
+      yp = bmul(fi, bsub(rk, bmul(fl, yp)));
+      yp.store(W + at * BB);
+    }
+    const Blk<B> ybot = yp;
+    Blk<B> ym;
+    {
+      const size_t at = AT(m);
+      Blk<B> fu;
+      fi.load(inv + at * BB);
+      fl.load(Lo + at * BB);
+      fu.load(Up + at * BB);
+      const Blk<B> rm = ((left && m == a) || (!left && m == b)) ? R : zero;
+      ym = bmul(fi, bsub(bsub(rm, bmul(fl, ytop)), bmul(fu, ybot)));
+      ym.store(W + at * BB);
+    }
+    Blk<B> wn = ym, yk, pk;
+    for (int k = m - 1; k >= a; --k) {
+      const size_t at = AT(k);
+      yk.load(W + at * BB);
+      pk.load(Up + at * BB);
+      wn = bsub(yk, bmul(pk, wn));
+      wn.store(W + at * BB);
+    }
+    wn = ym;
+    for (int k = m + 1; k <= b; ++k) {
+      const size_t at = AT(k);
+      yk.load(W + at * BB);
+      pk.load(Up + at * BB);
+      wn = bsub(yk, bmul(pk, wn));
+      wn.store(W + at * BB);
+    }
+  };
+  if (sg > 0) spike(WL, true);
+  else
+    for (int k = a; k <= b; ++k) zero.store(WL + AT(k) * BB);
+  if (sg < ns - 1) spike(WR, false);
+  else
+    for (int k = a; k <= b; ++k) zero.store(WR + AT(k) * BB);
+}
+
+// The separators' Schur system of each line with nseg[l] >= 2 (one thread
+// per line), factored by block Thomas: at separator k's position inv holds
+// the inverse pivot D'_k^{-1}, Lo the coupling to separator k - 1, Up
+// D'_k^{-1} times the coupling to separator k + 1, and WL / WR the
+// separator's own couplings to the cells before and after it.
+template <int B, class T>
+__global__ void k_line_reduce(int nl, const int* __restrict__ len, const int* __restrict__ nseg,
+                              const int* __restrict__ cell, const int* __restrict__ lo, const int* __restrict__ up,
+                              const int* __restrict__ diag, const double* __restrict__ val, T* __restrict__ inv,
+                              T* __restrict__ Lo, T* __restrict__ Up, T* __restrict__ WL, T* __restrict__ WR) {
+  constexpr int BB = B * B;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  const int L = len[l], ns = nseg[l];
+  if (L == 0 || ns < 2) return;
+  auto AT = [&](int k) -> size_t { return (size_t)k * nl + l; };
+  auto vblk = [&](int q) { Blk<B> x; x.load(val + (size_t)q * BB); return x; };
+  Blk<B> Uprev;
+  Uprev.zero();
+  for (int k = 0; k < ns - 1; ++k) {
+    const int sp = line_sep(L, ns, k);
+    const size_t at = AT(sp), atp = AT(sp - 1), atn = AT(sp + 1);
+    const Blk<B> Ass = vblk(diag[cell[at]]), Ls = vblk(lo[at]), Us = vblk(up[at]);
+    Blk<B> wrp, wln, wlp, wrn;
+    wrp.load(WR + atp * BB);
+    wln.load(WL + atn * BB);
+    wlp.load(WL + atp * BB);
+    wrn.load(WR + atn * BB);
+    const Blk<B> Tkk = bsub(bsub(Ass, bmul(Ls, wrp)), bmul(Us, wln));
+    Blk<B> Tsub = bmul(Ls, wlp), Tsup = bmul(Us, wrn);
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        Tsub.v[i][j] = -Tsub.v[i][j];
+        Tsup.v[i][j] = -Tsup.v[i][j];
+      }
+    const Blk<B> Dinv = binv(bsub(Tkk, bmul(Tsub, Uprev)));
+    Uprev = bmul(Dinv, Tsup);
+    Dinv.store(inv + at * BB);
+    Tsub.store(Lo + at * BB);
+    Uprev.store(Up + at * BB);
+    Ls.store(WL + at * BB);
+    Us.store(WR + at * BB);
+  }
+}
+
+// One colour's partitioned line solves: a wave holds 32 / NS lines, lanes
+// (line, segment) in both halves; lanes 0-31 walk their segments' top halves
+// downwards, lanes 32-63 the bottom halves upwards, as k_line_gs_tw walks
+// whole lines, keeping y and then g in LDS; one lane per line solves the
+// separators' system from the segments' end values; every lane then
+// corrects its cells by the spikes.  Lines shorter than the colour's longest
+// may have fewer segments; their spare lanes idle.  Outputs as k_line_gs_tw:
+// results by table position for the next colour, z by cell in the last
+// sweep (Z).  The dynamic LDS holds (jcap + 1) x B x 64 + 32 x B doubles.
+template <int B, class T, int D, int NS, bool Z>
+__global__ __launch_bounds__(64) void k_line_gs_seg(int nl, int lc0, int jcap, const int* __restrict__ len,
+                                                    const int* __restrict__ cell, const T* __restrict__ inv,
+                                                    const T* __restrict__ Lo, const T* __restrict__ Up,
+                                                    const T* __restrict__ WL, const T* __restrict__ WR,
+                                                    const T* __restrict__ r2l, const T* __restrict__ xpl,
+                                                    const PrecIO* __restrict__ io, T* __restrict__ ypos,
+                                                    const double* __restrict__ pcw) {
+  constexpr int BB = B * B, LPW = 32 / NS;
+  extern __shared__ __align__(16) unsigned char line_lds[];
+  double(*gs)[B][64] = reinterpret_cast<double(*)[B][64]>(line_lds);
+  double(*ends)[64] = reinterpret_cast<double(*)[64]>(line_lds + sizeof(double) * (size_t)jcap * B * 64);
+  double(*xs)[B] = reinterpret_cast<double(*)[B]>(line_lds + sizeof(double) * ((size_t)jcap + 1) * B * 64);
+  gmem<double>* __restrict__ z = global_ptr(io->z);
+  const int q = threadIdx.x, h = q >> 5, r = q & 31, li = r / NS, sg = r % NS;
+  const int l = lc0 + blockIdx.x * LPW + li;
+  if (len[lc0 + blockIdx.x * LPW] == 0) return;         // a wave of padding lines only (wave-uniform)
+  const int L = len[l];
+  const int ns = L > 0 ? line_nseg(L) : 1;
+  int a = 0, b = -1;
+  if (L > 0 && sg < ns) line_segment(L, ns, sg, a, b);
+  const int n = b - a + 1;                              // 0 on an idle lane
+  const int m = a + n / 2;
+  const int J = n <= 0 ? 0 : (h == 0 ? n / 2 : n - 1 - n / 2);
+  int Jw = J;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) Jw = max(Jw, __shfl_xor(Jw, o));
+  const int Jmax = (Jw + D - 1) / D * D;
+  auto pos = [&](int j) -> int {
+    const int jj = min(max(j, 0), max(J - 1, 0));
+    return n <= 0 ? 0 : (h == 0 ? a + jj : b - jj);
+  };
+  // forward walk: y into LDS, rounded to T as the walks through memory round it
+  T ri[D][BB], rl[D][BB], rr[D][B];
+  auto fload = [&](int j, int u) {
+    const size_t at = (size_t)pos(j) * nl + l;
+#pragma unroll
+    for (int k = 0; k < BB; ++k) {
+      ri[u][k] = inv[at * BB + k];
+      rl[u][k] = Lo[at * BB + k];
+    }
+#pragma unroll
+    for (int c = 0; c < B; ++c) rr[u][c] = r2l[at * B + c];
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) fload(u, u);
+  double y[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) y[c] = 0.0;
+  for (int j0 = 0; j0 < Jmax; j0 += D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 + u;
+      double rhs[B], yn[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        double v = (double)rr[u][c];
+#pragma unroll
+        for (int k = 0; k < B; ++k) v -= (double)rl[u][c * B + k] * y[k];
+        rhs[c] = v;
+      }
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < B; ++k) v += (double)ri[u][c * B + k] * rhs[k];
+        yn[c] = v;
+      }
+      if (j < J) {
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          y[c] = yn[c];
+          gs[j][c][q] = (double)(T)yn[c];
+        }
+      }
+      fload(j + D, u);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // the segment's middle cell
+  double yo[B], xm[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) {
+    yo[c] = __shfl_xor(y[c], 32);
+    xm[c] = 0.0;
+  }
+  if (n > 0) {
+    const size_t at = (size_t)m * nl + l;
+    double rhs[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      double v = (double)r2l[at * B + c];
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const double yt = h == 0 ? y[k] : yo[k], yb = h == 0 ? yo[k] : y[k];
+        v -= (double)Lo[at * BB + c * B + k] * yt + (double)Up[at * BB + c * B + k] * yb;
+      }
+      rhs[c] = v;
+    }
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      double v = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) v += (double)inv[at * BB + c * B + k] * rhs[k];
+      xm[c] = v;
+    }
+  }
+  // backward walk: g = y - P g_next, into LDS
+  T bp[D][BB];
+  auto bload = [&](int j, int u) {
+    const size_t at = (size_t)pos(j) * nl + l;
+#pragma unroll
+    for (int k = 0; k < BB; ++k) bp[u][k] = Up[at * BB + k];
+  };
+#pragma unroll
+  for (int u = 0; u < D; ++u) bload(Jmax - 1 - u, u);
+  double xn[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) xn[c] = xm[c];
+  for (int j0 = Jmax - 1; j0 >= 0; j0 -= D) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 - u;
+      double tq[B];
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < B; ++k) v += (double)bp[u][c * B + k] * xn[k];
+        tq[c] = v;
+      }
+      if (j < J) {
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          xn[c] = gs[j][c][q] - tq[c];
+          gs[j][c][q] = xn[c];
+        }
+      }
+      bload(j - D, u);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // end values: the top lane ends at the segment's first cell, the bottom
+  // lane at its last (the middle when the half is empty)
+#pragma unroll
+  for (int c = 0; c < B; ++c) ends[c][q] = J > 0 ? xn[c] : xm[c];
+  __syncthreads();
+  double pc[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) pc[c] = pcw[c];
+  // the separators of each line, one lane per line
+  if (h == 0 && sg == 0 && L > 0 && ns > 1) {
+    double zk[NS - 1][B], zp[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) zp[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS - 1; ++k) {
+      if (k < ns - 1) {
+        const size_t at = (size_t)line_sep(L, ns, k) * nl + l;
+        const int ql = 32 + li * NS + k, qf = li * NS + k + 1;   // segment k's bottom lane, k+1's top lane
+        double rhs[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          double v = (double)r2l[at * B + c];
+#pragma unroll
+          for (int e = 0; e < B; ++e)
+            v -= (double)WL[at * BB + c * B + e] * ends[e][ql] + (double)WR[at * BB + c * B + e] * ends[e][qf] +
+                 (double)Lo[at * BB + c * B + e] * zp[e];
+          rhs[c] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          double v = 0.0;
+#pragma unroll
+          for (int e = 0; e < B; ++e) v += (double)inv[at * BB + c * B + e] * rhs[e];
+          zk[k][c] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) zp[c] = zk[k][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < B; ++c) zp[c] = 0.0;
+#pragma unroll
+    for (int k = NS - 2; k >= 0; --k) {
+      if (k < ns - 1) {
+        const size_t at = (size_t)line_sep(L, ns, k) * nl + l;
+        double x[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          double v = zk[k][c];
+#pragma unroll
+          for (int e = 0; e < B; ++e) v -= (double)Up[at * BB + c * B + e] * zp[e];
+          x[c] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < B; ++c) {
+          zp[c] = x[c];
+          xs[li * NS + k][c] = x[c];
+          if (ypos != nullptr) ypos[at * B + c] = (T)x[c];
+        }
+        if (Z) {
+          const int cc = cell[at];
+          const double xp = (double)xpl[at];
+          if (cc >= 0)
+#pragma unroll
+            for (int c = 0; c < B; ++c) z[(size_t)cc * B + c] = x[c] + pc[c] * xp;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // correction of the segment's cells by its separators
+  if (n <= 0) return;
+  const bool hl = sg > 0, hr = sg < ns - 1;
+  double xl[B], xr[B];
+#pragma unroll
+  for (int c = 0; c < B; ++c) {
+    xl[c] = hl ? xs[li * NS + sg - 1][c] : 0.0;
+    xr[c] = hr ? xs[li * NS + sg][c] : 0.0;
+  }
+  auto finish = [&](size_t at, const double (&g)[B], const T (&wl)[BB], const T (&wr)[BB], T xp, int cc) {
+    double x[B];
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      double v = g[c];
+#pragma unroll
+      for (int e = 0; e < B; ++e) v -= (double)wl[c * B + e] * xl[e] + (double)wr[c * B + e] * xr[e];
+      x[c] = v;
+    }
+    if (ypos != nullptr)
+#pragma unroll
+      for (int c = 0; c < B; ++c) ypos[at * B + c] = (T)x[c];
+    if (Z && cc >= 0)
+#pragma unroll
+      for (int c = 0; c < B; ++c) z[(size_t)cc * B + c] = x[c] + pc[c] * (double)xp;
+  };
+  for (int j0 = 0; j0 < Jmax; j0 += D) {
+    T wl[D][BB], wr[D][BB], ox[D];
+    int oc[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const size_t at = (size_t)pos(j0 + u) * nl + l;
+#pragma unroll
+      for (int k = 0; k < BB; ++k) {           // zero on the sides without a separator
+        wl[u][k] = WL[at * BB + k];
+        wr[u][k] = WR[at * BB + k];
+      }
+      ox[u] = Z ? xpl[at] : (T)0;
+      oc[u] = Z ? cell[at] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int j = j0 + u;
+      if (j < J) {
+        double g[B];
+#pragma unroll
+        for (int c = 0; c < B; ++c) g[c] = gs[j][c][q];
+        finish((size_t)pos(j) * nl + l, g, wl[u], wr[u], ox[u], oc[u]);
+      }
+    }
+  }
+  if (h == 0) {
+    const size_t at = (size_t)m * nl + l;
+    T wl[BB], wr[BB];
+#pragma unroll
+    for (int k = 0; k < BB; ++k) {
+      wl[k] = WL[at * BB + k];
+      wr[k] = WR[at * BB + k];
+    }
+    finish(at, xm, wl, wr, Z ? xpl[at] : (T)0, Z ? cell[at] : -1);
   }
 }
 
@@ -1761,6 +2303,11 @@ class CprSolver {
   double lgs_bytes = 0.0;
   bool capturing = false;
   std::vector<long long> color_cells;                // cells on the lines of each colour
+  // partitioned lines: segments per line of each colour's longest line (1:
+  // whole lines) and the longest segment half-walk
+  std::vector<int> color_ns, color_jcap;
+  bool seg_active = false;                           // this setup factored the partitioned lines
+  DArr<int> ln_nseg;                                 // segments of each line (0: whole-line factors)
   // line local stage: position-major line tables and block-tridiagonal factors
   int nlines = 0, maxlen = 0, longest = 0, ncolor = 0;
   std::vector<int> color_ptr, color_len;             // line range and longest line of each colour
@@ -1770,8 +2317,8 @@ class CprSolver {
   int noff = 0;                                      // off-line coupling planes
   DArr<int> ln_pos, off_q, off_nbr;
   DArr<unsigned char> off_col;
-  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64, aoff64;
-  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, aoff32;
+  DArr<double> lf_inv64, lf_lo64, lf_up64, r2_64, xpl64, yc64, r1l64, aoff64, lf_wl64, lf_wr64;
+  DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, aoff32, lf_wl32, lf_wr32;
   // coarse levels handled by one workgroup: first such level, descriptions
   int coarse_l0 = -1;
   DArr<char> coarse_desc;
@@ -1927,6 +2474,35 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
     for (int l = HL.color_ptr[c]; l < HL.color_ptr[c + 1]; ++l) C->color_cells[c] += HL.len[l];
   for (int c = 0; c < HL.ncolor; ++c)
     if (HL.color_ptr[c + 1] > HL.color_ptr[c]) C->color_len[c] = HL.len[HL.color_ptr[c]];
+  // partitioned lines: a colour whose longest line has 32 cells or more is
+  // solved by segments, unless its longest half-walk would not fit the
+  // 64 KiB of LDS a workgroup takes (3 x 3 blocks: 41 positions)
+  {
+    const int nc_ = HL.ncolor;
+    C->color_ns.assign(nc_, 1);
+    C->color_jcap.assign(nc_, 0);
+    std::vector<int> nseg(HL.nlines, 0);
+    for (int c = 0; c < nc_; ++c) {
+      const int NS = line_nseg(C->color_len[c]);
+      if (NS < 2) continue;
+      int jcap = 0;
+      for (int l = HL.color_ptr[c]; l < HL.color_ptr[c + 1]; ++l) {
+        const int L = HL.len[l];
+        if (L == 0) continue;
+        const int ns = line_nseg(L);
+        for (int k = 0; k < ns; ++k) {
+          int a, b;
+          line_segment(L, ns, k, a, b);
+          jcap = std::max(jcap, (b - a + 1) / 2);
+        }
+      }
+      if (jcap > 41) continue;
+      C->color_ns[c] = NS;
+      C->color_jcap[c] = std::max(jcap, 1);
+      for (int l = HL.color_ptr[c]; l < HL.color_ptr[c + 1]; ++l) nseg[l] = HL.len[l] > 0 ? line_nseg(HL.len[l]) : 0;
+    }
+    C->ln_nseg.upload(nseg, s);
+  }
   C->ln_len.upload(HL.len, s);
   C->ln_cell.upload(HL.cell, s);
   C->ln_lo.upload(HL.lo, s);
@@ -1999,6 +2575,8 @@ struct Lv<double> {
   static DArr<double>& lf_inv(CprSolver& c) { return c.lf_inv64; }
   static DArr<double>& lf_lo(CprSolver& c) { return c.lf_lo64; }
   static DArr<double>& lf_up(CprSolver& c) { return c.lf_up64; }
+  static DArr<double>& lf_wl(CprSolver& c) { return c.lf_wl64; }
+  static DArr<double>& lf_wr(CprSolver& c) { return c.lf_wr64; }
   static DArr<double>& r2(CprSolver& c) { return c.r2_64; }
   static DArr<double>& xpl(CprSolver& c) { return c.xpl64; }
   static DArr<double>& yc(CprSolver& c) { return c.yc64; }
@@ -2026,6 +2604,8 @@ struct Lv<float> {
   static DArr<float>& lf_inv(CprSolver& c) { return c.lf_inv32; }
   static DArr<float>& lf_lo(CprSolver& c) { return c.lf_lo32; }
   static DArr<float>& lf_up(CprSolver& c) { return c.lf_up32; }
+  static DArr<float>& lf_wl(CprSolver& c) { return c.lf_wl32; }
+  static DArr<float>& lf_wr(CprSolver& c) { return c.lf_wr32; }
   static DArr<float>& r2(CprSolver& c) { return c.r2_32; }
   static DArr<float>& xpl(CprSolver& c) { return c.xpl32; }
   static DArr<float>& yc(CprSolver& c) { return c.yc32; }
@@ -2087,10 +2667,25 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     V::lf_up(*C).zero(s);
     V::r2(*C).zero(s);
     V::xpl(*C).zero(s);
+    C->seg_active = o.line_segments != 0 &&
+                    std::any_of(C->color_ns.begin(), C->color_ns.end(), [](int v) { return v > 1; });
     hipLaunchKernelGGL((k_line_factor_tw<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
-                       C->ln_len.p, C->ln_cell.p, C->ln_lo.p, C->ln_up.p, S.diag.p, S.val.p, V::lf_inv(*C).p,
-                       V::lf_lo(*C).p, V::lf_up(*C).p);
+                       C->ln_len.p, C->seg_active ? C->ln_nseg.p : nullptr, C->ln_cell.p, C->ln_lo.p, C->ln_up.p,
+                       S.diag.p, S.val.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p);
     launch_check();
+    if (C->seg_active) {
+      V::lf_wl(*C).alloc(m);
+      V::lf_wr(*C).alloc(m);
+      hipLaunchKernelGGL((k_line_factor_seg<B, T>), dim3(grid_for((size_t)C->nlines * 8, 64)), dim3(64), 0, s,
+                         C->nlines, C->ln_len.p, C->ln_nseg.p, C->ln_cell.p, C->ln_lo.p, C->ln_up.p, S.diag.p,
+                         S.val.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::lf_wl(*C).p,
+                         V::lf_wr(*C).p);
+      launch_check();
+      hipLaunchKernelGGL((k_line_reduce<B, T>), dim3(grid_for(C->nlines, 64)), dim3(64), 0, s, C->nlines,
+                         C->ln_len.p, C->ln_nseg.p, C->ln_cell.p, C->ln_lo.p, C->ln_up.p, S.diag.p, S.val.p,
+                         V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::lf_wl(*C).p, V::lf_wr(*C).p);
+      launch_check();
+    }
   }
   if (o.local_stage == 2) {
     const size_t nb = (size_t)S.nnzb * B * B, nd = (size_t)S.nc * B * B;
@@ -2390,6 +2985,38 @@ static void stage2(CprSolver* C, const BlockSystem& S, const LinearSolverOptions
         const bool zs = sw + 1 == sweeps;               // the last sweep writes the output
         hipEvent_t e0 = nullptr, e1 = nullptr;
         const bool timed = !C->capturing && C->lgs_t.pair(&e0, &e1);
+        const int NS = C->seg_active ? C->color_ns[c] : 1;
+        if (NS > 1) {
+          const int jcap = C->color_jcap[c];
+          const size_t lds = sizeof(double) * ((size_t)(jcap + 1) * B * 64 + 32 * B);
+          decltype(&k_line_gs_seg<B, T, D, 2, true>) sk;
+          if (NS == 2) sk = zs ? k_line_gs_seg<B, T, D, 2, true> : k_line_gs_seg<B, T, D, 2, false>;
+          else if (NS == 4) sk = zs ? k_line_gs_seg<B, T, D, 4, true> : k_line_gs_seg<B, T, D, 4, false>;
+          else sk = zs ? k_line_gs_seg<B, T, D, 8, true> : k_line_gs_seg<B, T, D, 8, false>;
+          const dim3 grid(nlc / 32 * NS);
+          if (timed)
+            hipExtLaunchKernelGGL(sk, grid, dim3(64), lds, s, e0, e1, 0, C->nlines, lc0, jcap, C->ln_len.p,
+                                  C->ln_cell.p, V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::lf_wl(*C).p,
+                                  V::lf_wr(*C).p, V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p,
+                                  C->pcdev);
+          else
+            hipLaunchKernelGGL(sk, grid, dim3(64), lds, s, C->nlines, lc0, jcap, C->ln_len.p, C->ln_cell.p,
+                               V::lf_inv(*C).p, V::lf_lo(*C).p, V::lf_up(*C).p, V::lf_wl(*C).p, V::lf_wr(*C).p,
+                               V::r2(*C).p, V::xpl(*C).p, C->io, last ? nullptr : V::yc(*C).p, C->pcdev);
+          launch_check();
+          if (timed) {
+            // per line cell five blocks and r read (a segment cell: D'^-1, L
+            // and r going forward, P going back and its two spikes, zero at a
+            // line's ends; a separator: its two couplings and three Thomas
+            // blocks), the result written by table position unless this is
+            // the last launch, and in the last sweep xp and the cell index
+            // read and z written
+            const double sv = sizeof(T), bb = B * B * sv;
+            C->lgs_bytes += (double)C->color_cells[c] *
+                            (5.0 * bb + B * sv + (last ? 0.0 : B * sv) + (zs ? sv + 4.0 + 8.0 * B : 0.0));
+          }
+          continue;
+        }
         auto kern = zs ? k_line_gs_tw<B, T, D, true> : k_line_gs_tw<B, T, D, false>;
         if (timed)
           hipExtLaunchKernelGGL(kern, dim3(nlc / 32), dim3(64), 0, s, e0, e1, 0, C->nlines, lc0, C->ln_len.p,
@@ -2449,7 +3076,7 @@ static bool same_apply_options(const LinearSolverOptions& a, const LinearSolverO
   return a.precision == b.precision && a.use_pressure_stage == b.use_pressure_stage &&
          a.pre_smooth == b.pre_smooth && a.post_smooth == b.post_smooth && a.coarse_alpha == b.coarse_alpha &&
          a.amg_cycles == b.amg_cycles && a.local_stage == b.local_stage && a.local_sweeps == b.local_sweeps &&
-         a.coarse_block_rows == b.coarse_block_rows;
+         a.coarse_block_rows == b.coarse_block_rows && a.line_segments == b.line_segments;
 }
 
 // Capture the application once per setup (on a private stream: nothing runs

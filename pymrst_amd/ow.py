@@ -115,6 +115,7 @@ class CPRSolver:
     local_sweeps: int = 4
     use_graphs: bool = True
     coarse_block_rows: int = 0
+    line_segments: bool = False
     # a pressure-stage solver with solve_linear_system(A_p, r) -> x, called on
     # the equilibrated A_p = W J P in every preconditioner application
     # instead of the device AMG (linear_solvers.py:124-131, 253-256)
@@ -166,6 +167,7 @@ class CPRSolver:
         o.local_sweeps = self.local_sweeps
         o.use_graphs = int(self.use_graphs)
         o.coarse_block_rows = self.coarse_block_rows
+        o.line_segments = 1 if self.line_segments else 0
         return o
 
 

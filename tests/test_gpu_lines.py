@@ -166,3 +166,107 @@ def test_line_stage_application_matches_coloured_line_gauss_seidel(sweeps):
         idx = 2 * nc + np.arange(3) * nw + w
         zw = np.linalg.solve(Jd[idx][:, idx].toarray(), rh[2 * nc + 3 * w:2 * nc + 3 * w + 3])
         assert np.abs(zh[2 * nc + 3 * w:2 * nc + 3 * w + 3] - zw).max() <= 1e-9 * np.abs(zw).max(), w
+
+
+def _line_stage_application(nx, ny, nz, lines, sweeps, seed=5, segments=False):
+    """One application of the line local stage alone (FP64, no pressure
+    stage, no graph replay) on the given lines, and the dense coloured line
+    Gauss-Seidel restatement of it on the exported Jacobian blocks."""
+    from pymrst_amd import _lib
+    from pymrst_amd.ow import CPRSolver, DeviceOWState
+    inp = small_gravity_inputs(nx=nx, ny=ny, nz=nz)
+    mo, so = build_oracle(inp)
+    st = so.copy()
+    rng = np.random.default_rng(seed)
+    st.pressure = st.pressure + rng.normal(0, 2e5, mo.nc)
+    st.sW = np.clip(st.sW + rng.normal(0, 0.1, mo.nc), 0.0, 1.0)
+    dt = 10 * og.DAY
+    cdp = mo.compute_cdp_per_well(so)
+    solver = CPRSolver(rtol=1e-10, precision=64, local_sweeps=sweeps, use_pressure_stage=False, use_graphs=False,
+                       line_segments=segments)
+    md, _ = build_device(inp, solver)
+    if lines is not None:
+        ptr = np.ascontiguousarray(np.concatenate([[0], np.cumsum([len(s) for s in lines])]), dtype=np.int32)
+        cells = np.ascontiguousarray(np.concatenate(lines), dtype=np.int32)
+        _lib.check(md.lib.pmx_ow_set_lines(md.h, len(lines), _lib.ptr(ptr), _lib.ptr(cells)), "pmx_ow_set_lines")
+    else:
+        lines = [np.array([i + nx * (j + ny * k) for k in range(nz)]) for j in range(ny) for i in range(nx)]
+    dev = torch.device("cuda")
+    md.assemble_device(DeviceOWState.from_host(so, dev), DeviceOWState.from_host(st, dev), dt,
+                       torch.as_tensor(np.concatenate(cdp), device=dev))
+    ok, pmean = md.converged(dt, DeviceOWState.from_host(st, dev))
+    opts = solver.options()
+    info = _lib.SolveInfo()
+    rc = md.lib.pmx_ow_solve(md.h, ctypes.byref(opts), pmean, _lib.ptr(md._dx), ctypes.byref(info))
+    assert rc in (0, 1), rc
+    nc, nw = mo.nc, len(mo.wells)
+    r = torch.as_tensor(np.random.default_rng(9).normal(size=2 * nc + 3 * nw), device=dev)
+    z = torch.empty_like(r)
+    _lib.check(md.lib.pmx_ow_precondition(md.h, ctypes.byref(opts), _lib.ptr(r), _lib.ptr(z)),
+               "pmx_ow_precondition")
+    nnzb = ctypes.c_int32()
+    _lib.check(md.lib.pmx_ow_nnzb(md.h, ctypes.byref(nnzb)))
+    rowptr = np.zeros(nc + 1, np.int32)
+    col = np.zeros(nnzb.value, np.int32)
+    _lib.check(md.lib.pmx_ow_export_pattern(md.h, _lib.ptr(rowptr), _lib.ptr(col)))
+    vals = torch.empty(nnzb.value * 4, dtype=torch.float64, device=dev)
+    _lib.check(md.lib.pmx_ow_export_blocks(md.h, _lib.ptr(vals)))
+    blocks = vals.cpu().numpy().reshape(-1, 2, 2)
+    rh, zh = r.cpu().numpy(), z.cpu().numpy()
+    y = _coloured_line_gauss_seidel(rowptr, col, blocks, lines, rh[:2 * nc].reshape(nc, 2), sweeps)
+    return zh[:2 * nc], y.ravel()
+
+
+@pytest.mark.parametrize("segments", [True, False])
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("cut", [False, True])
+def test_long_line_stage_matches_coloured_line_gauss_seidel(cut, sweeps, segments):
+    """Columns of 85 cells, each solved by one twisted walk or (segments)
+    by four segments cut at three separator cells, with spikes and a Schur
+    system on the separators; the cut variant also makes lines of 1 to 85
+    cells, so segmented lines of 32-63 cells (two segments) and whole lines
+    share waves and colours.  One application equals the dense coloured
+    block line Gauss-Seidel to 1e-11 (FP64): both are exact line solves."""
+    nx, ny, nz = 6, 5, 85
+    lines = _irregular_lines(nx, ny, nz, seed=4) if cut else None
+    if cut:
+        lengths = [len(s) for s in lines]
+        assert max(lengths) >= 64 and any(32 <= n < 64 for n in lengths) and min(lengths) < 32
+    print(f"long line stage: cut={cut} sweeps={sweeps} segments={segments}", flush=True)
+    z, y = _line_stage_application(nx, ny, nz, lines, sweeps, segments=segments)
+    err = np.abs(z - y).max() / np.abs(y).max()
+    assert err <= 1e-11, f"cut={cut} sweeps={sweeps}: line stage differs from line Gauss-Seidel by {err:.2e}"
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_partitioned_lines_solve_matches_direct(prec):
+    """CPR-FGMRES with the partitioned line stage on 85-cell columns solves a
+    perturbed Newton system to the direct solution within 1e-7 relative
+    (the reference's CPR contract, linear_solvers.py:639-649)."""
+    from pymrst_amd import _lib
+    from pymrst_amd.ow import CPRSolver, DeviceOWState
+    nx, ny, nz = 6, 5, 85
+    inp = small_gravity_inputs(nx=nx, ny=ny, nz=nz)
+    mo, so = build_oracle(inp)
+    st = so.copy()
+    rng = np.random.default_rng(13)
+    st.pressure = st.pressure + rng.normal(0, 2e5, mo.nc)
+    st.sW = np.clip(st.sW + rng.normal(0, 0.1, mo.nc), 0.0, 1.0)
+    dt = 10 * og.DAY
+    cdp = mo.compute_cdp_per_well(so)
+    r_ref, J_ref, _ = mo.residual_and_jacobian(so, st, dt, cdp)
+    x_ref = spla.spsolve(J_ref.tocsc(), -r_ref)
+    perm = device_to_reference_perm(mo.nc, len(mo.wells))
+    md, _ = build_device(inp, CPRSolver(rtol=1e-10, precision=prec, line_segments=True))
+    dev = torch.device("cuda")
+    md.assemble_device(DeviceOWState.from_host(so, dev), DeviceOWState.from_host(st, dev), dt,
+                       torch.as_tensor(np.concatenate(cdp), device=dev))
+    ok, pmean = md.converged(dt, DeviceOWState.from_host(st, dev))
+    info = _lib.SolveInfo()
+    print(f"partitioned lines CPR solve FP{prec}", flush=True)
+    rc = md.lib.pmx_ow_solve(md.h, ctypes.byref(md.linear_solver.options()), pmean, _lib.ptr(md._dx),
+                             ctypes.byref(info))
+    assert rc == 0, f"FP{prec}: not converged ({info.iterations} its, rel {info.rel_residual:.1e})"
+    x = md._dx.cpu().numpy()
+    err = float(np.abs(x - x_ref[perm]).max() / np.abs(x_ref).max())
+    assert err <= 1e-7, f"FP{prec}: CPR is not the direct solution: {err:.2e} ({info.iterations} its)"
