@@ -98,9 +98,35 @@ int pmx_ow_residual(pmx_ow* h, double* out);
  * wells_out (host, 3*nw) = well rows of the residual. */
 int pmx_ow_convergence(pmx_ow* h, const double* p, double* out7, double* wells_out);
 
+/* The facility's controls (FacilityModel._control_equation,
+ * pymrst/modules/ad_core/models/facilities/facility_model.py:2589-2658, and
+ * _single_well :2434-2549): each well's control row is the linear form
+ * coef[3w] qWs + coef[3w+1] qOs + coef[3w+2] bhp - val[w] (a bhp target,
+ * a total, single-phase or liquid rate, a zero target's total rate, a
+ * reservoir-rate form with control densities); status[w] = 0 shuts a well
+ * (rows qWs = qOs = 0 and bhp = the mean of its perforations' pressures);
+ * the two rate-closure rows of every open well are scaled by rscale[2]
+ * (MRST's rhoScale).  Host arrays; applies from the next pmx_ow_assemble.
+ * pmx_ow_set_wells resets them to its bhp / rate controls, unscaled. */
+int pmx_ow_set_controls(pmx_ow* h, const double* coef, const double* val, const int32_t* status,
+                        const double* rscale);
+
+/* pmx_ow_convergence with the current pore volume pv (1 + c_r (p - p_ref))
+ * in the maxima (GenericBlackOilModelAD._convergence_scaling,
+ * pymrst/modules/ad_blackoil/models/generic_black_oil_model.py:604-638) and
+ * its sum as out8[7]. */
+int pmx_ow_convergence_current_pv(pmx_ow* h, const double* p, double* out8, double* wells_out);
+
+/* Reservoir volume rates of every perforation at the last pmx_ow_assemble
+ * (host, 2*nperf, [water, oil] per perforation: a producing perforation's
+ * phase rates, an injecting one's total rate times compi), the facility's
+ * cq_vol (facility_model.py:2512-2528) that a converged step hands to the
+ * next one's wellbore column (generic_black_oil_model.py:569-601). */
+int pmx_ow_perf_volume_rates(pmx_ow* h, double* out);
+
 /* pmx_ow_convergence without any allocation: the same reductions and
  * outputs, with the partial sums kept in a static per-GPU array of the
- * library (512 rows of 7 doubles plus the 7 results, 28,736 bytes, held
+ * library (512 rows of 8 doubles plus the 8 results, 32,832 bytes, held
  * until the process ends) instead of handle memory.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops).  Every call on one GPU shares

@@ -193,6 +193,8 @@ class Well:
     compi: np.ndarray
     sign: int = 0
     name: str = ""
+    lims: Optional[dict] = None      # limits a controlled well may switch to (types.py:42-72)
+    status: bool = True              # an open well
 
 
 def well_index(g: CartGrid, rock: Rock, cells, radius=0.1, skin=0.0, well_dir="z"):
@@ -216,7 +218,7 @@ def well_index(g: CartGrid, rock: Rock, cells, radius=0.1, skin=0.0, well_dir="z
 
 def add_well(wells: Optional[List[Well]], g: CartGrid, rock: Rock, cells, well_type="bhp",
              val=0.0, radius=0.1, compi=None, sign=0, gravity=None, name=None,
-             ref_depth=None, well_dir="z") -> List[Well]:
+             ref_depth=None, well_dir="z", lims=None, status=True) -> List[Well]:
     """pymrst/params/wells_and_bc/add_well.py:16-126 (well_dir x, y or z).
 
     dZ = depth - ref_depth where depth is the cell centroid along the
@@ -235,5 +237,6 @@ def add_well(wells: Optional[List[Well]], g: CartGrid, rock: Rock, cells, well_t
         sign = int(np.sign(val))
     wells.append(Well(cells=cells, type=well_type, val=float(val), WI=wi,
                       dZ=depth - ref_depth, compi=compi, sign=int(sign),
-                      name=name or f"W{len(wells) + 1}"))
+                      name=name or f"W{len(wells) + 1}", lims=dict(lims) if lims else None,
+                      status=bool(status)))
     return wells

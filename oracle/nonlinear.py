@@ -106,6 +106,8 @@ class StabilisedNewton:
             history.append(res)
             norm = max(res.values())
             if ok:
+                if hasattr(model, "after_convergence"):     # nonlinear_solver.py:282-288
+                    st = model.after_convergence(st, state0, dt)
                 return st, True, it, searches
             struggling = self.always_use_stabilization or self.issues
             if self.use_relaxation:

@@ -66,6 +66,9 @@ SIGNATURES = {
     "pmx_ow_residual": (ctypes.c_int, [_P, _P]),
     "pmx_ow_convergence": (ctypes.c_int, [_P, _P, _P, _P]),
     "pmx_ow_convergence_noalloc": (ctypes.c_int, [_P, _P, _P, _P]),
+    "pmx_ow_set_controls": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "pmx_ow_convergence_current_pv": (ctypes.c_int, [_P, _P, _P, _P]),
+    "pmx_ow_perf_volume_rates": (ctypes.c_int, [_P, _P]),
     "pmx_ow_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _D, _P, ctypes.POINTER(SolveInfo)]),
     "pmx_ow_update": (ctypes.c_int, [_P, _P, _D, _D, _D, _P, _P, _P, _P, _P]),
     "pmx_ow_matvec": (ctypes.c_int, [_P, _P, _P]),

@@ -205,6 +205,40 @@ int pmx_ow_convergence(pmx_ow* h, const double* p, double* out7, double* wells_o
   })
 }
 
+int pmx_ow_set_controls(pmx_ow* h, const double* coef, const double* val, const int32_t* status,
+                        const double* rscale) {
+  PMX_TRY({
+    PMX_CHECK(h && rscale, "null argument");
+    bind(h);
+    auto& m = *h->m;
+    PMX_CHECK(m.nw == 0 || (coef && val && status), "null well control argument");
+    for (int w = 0; w < m.nw; ++w) PMX_CHECK(status[w] == 0 || status[w] == 1, "well status is 0 (shut) or 1");
+    PMX_CHECK(std::isfinite(rscale[0]) && std::isfinite(rscale[1]) && rscale[0] > 0 && rscale[1] > 0,
+              "rate-row scales must be positive");
+    m.set_controls(coef, val, status, rscale);
+  })
+}
+
+int pmx_ow_convergence_current_pv(pmx_ow* h, const double* p, double* out8, double* wells_out) {
+  PMX_TRY({
+    PMX_CHECK(h && p && out8, "null argument");
+    bind(h);
+    h->m->convergence_current_pv(p, out8, wells_out);
+  })
+}
+
+int pmx_ow_perf_volume_rates(pmx_ow* h, double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && out, "null argument");
+    bind(h);
+    auto& m = *h->m;
+    if (m.S.nperf > 0) {
+      PMX_HIP(hipMemcpyAsync(out, m.qvol.p, sizeof(double) * 2 * m.S.nperf, hipMemcpyDeviceToHost, m.stream));
+      PMX_HIP(hipStreamSynchronize(m.stream));
+    }
+  })
+}
+
 int pmx_ow_convergence_noalloc(pmx_ow* h, const double* p, double* out7, double* wells_out) {
   PMX_TRY({
     PMX_CHECK(h && p && out7, "null argument");

@@ -21,11 +21,17 @@ class OWModelDev {
                  const double* vals, const double* compi);
   void assemble(const double* p, const double* s, const double* qws, const double* qos, const double* bhp,
                 const double* p0, const double* s0, double dt, const double* cdp_dev);
+  // controls as linear forms of the well unknowns (coef[3 w..] on qWs, qOs,
+  // bhp; val the target), open/shut status and the rate-row scales
+  void set_controls(const double* coef, const double* vals, const int* status, const double* rscale);
   void convergence(const double* p, double* out7, double* wells_out);
+  // the same with the current pore volume and its sum as an eighth value
+  void convergence_current_pv(const double* p, double* out8, double* wells_out);
   // the same reductions with the partials in a static per-GPU array: no
   // allocation on any call (device: the GPU this model runs on, current)
   void convergence_noalloc(int device, const double* p, double* out7, double* wells_out);
-  void reduce_convergence(const double* p, double* out7, double* wells_out, double* partial, int nblk);
+  void reduce_convergence(const double* p, double* out, int nout, double* wells_out, double* partial, int nblk,
+                          int cur_pv);
   int solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean);
   void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
               double* qos, double* bhp);
@@ -47,9 +53,9 @@ class OWModelDev {
   BlockSystem S;
   DArr<int> bconn;
   DArr<double> bT, bG, pv;
-  DArr<double> WI, cdp, wval, compi, rq, dq;
-  DArr<int> wtype, mixflag;
-  std::vector<int> host_types;
+  DArr<double> WI, cdp, wval, compi, rq, dq, wctrl, qvol;
+  DArr<int> wstat, mixflag;
+  double rscale[2] = {1.0, 1.0};
   DArr<double> r, rhs, w_cpr, conv_partial;
   std::vector<double> hier_weights;
   std::vector<int> host_bconn, line_ptr, line_cells;

@@ -103,18 +103,48 @@ __global__ __launch_bounds__(256) void k_ow_cells(int nc, const int* __restrict_
 }
 
 // One 64-wide block per well: connection rates, well equations and all
-// derivative pieces of the well model (mixture terms included).
+// derivative pieces of the well model (mixture terms included).  The control
+// row is ctrl = c_w qWs + c_o qOs + c_b bhp - val (wctrl[3 w + .] = c_w, c_o,
+// c_b: a bhp target, a total or single-phase rate, or any linear form of the
+// well's own unknowns); the two rate-closure rows are scaled by rs_w and
+// rs_o.  A shut well (wstat = 0) takes no flux: its rows are q = 0 and bhp =
+// the mean of its perforations' pressures.  qvol receives each perforation's
+// reservoir volume rates (producing: the phase rates; injecting: the total
+// times compi).
 __global__ __launch_bounds__(64) void k_ow_wells(
     int nw, const int* __restrict__ well_ptr, const int* __restrict__ perf_cell, const double* __restrict__ WI,
-    const double* __restrict__ cdp, const int* __restrict__ wtype, const double* __restrict__ wval,
-    const double* __restrict__ compi, const double* __restrict__ p, const double* __restrict__ s,
-    const double* __restrict__ qws, const double* __restrict__ qos, const double* __restrict__ bhp, OWFluid f,
-    size_t nc2, double* __restrict__ r, double* __restrict__ rq, double* __restrict__ dq, double* __restrict__ Bp,
-    double* __restrict__ Cp, double* __restrict__ Um, double* __restrict__ V, double* __restrict__ Dp,
-    int* __restrict__ mixflag) {
+    const double* __restrict__ cdp, const double* __restrict__ wctrl, const int* __restrict__ wstat,
+    const double* __restrict__ wval, const double* __restrict__ compi, const double* __restrict__ p,
+    const double* __restrict__ s, const double* __restrict__ qws, const double* __restrict__ qos,
+    const double* __restrict__ bhp, OWFluid f, double rs_w, double rs_o, size_t nc2, double* __restrict__ r,
+    double* __restrict__ rq, double* __restrict__ dq, double* __restrict__ Bp, double* __restrict__ Cp,
+    double* __restrict__ Um, double* __restrict__ V, double* __restrict__ Dp, int* __restrict__ mixflag,
+    double* __restrict__ qvol) {
   const int w = blockIdx.x, lane = threadIdx.x;
   const int j0 = well_ptr[w], j1 = well_ptr[w + 1];
   const double bh = bhp[w], qw = qws[w], qo = qos[w];
+  if (wstat[w] == 0) {
+    const double inv_n = j1 > j0 ? 1.0 / (double)(j1 - j0) : 0.0;
+    double psum = 0.0;
+    for (int j = j0 + lane; j < j1; j += 64) {
+      psum += p[perf_cell[j]];
+      rq[2 * (size_t)j] = rq[2 * (size_t)j + 1] = 0.0;
+      qvol[2 * (size_t)j] = qvol[2 * (size_t)j + 1] = 0.0;
+      for (int q = 0; q < 4; ++q) dq[4 * (size_t)j + q] = Um[4 * (size_t)j + q] = V[4 * (size_t)j + q] = 0.0;
+      for (int q = 0; q < 6; ++q) Bp[6 * (size_t)j + q] = Cp[6 * (size_t)j + q] = 0.0;
+      Cp[6 * (size_t)j + 4] = -inv_n;
+    }
+    psum = lane_sum(psum, 64);
+    if (lane == 0) {
+      r[nc2 + 3 * (size_t)w + 0] = qw;
+      r[nc2 + 3 * (size_t)w + 1] = qo;
+      r[nc2 + 3 * (size_t)w + 2] = bh - psum * inv_n;
+      double* D = Dp + 9 * (size_t)w;
+      for (int q = 0; q < 9; ++q) D[q] = (q % 4 == 0) ? 1.0 : 0.0;
+      mixflag[w] = 0;
+    }
+    return;
+  }
   // Phase A: producing connections
   double Sw = 0, So = 0, dSw_b = 0, dSo_b = 0;
   int anyprod = 0;
@@ -211,6 +241,14 @@ __global__ __launch_bounds__(64) void k_ow_wells(
     dQo_b += lo[2];
     rq[2 * (size_t)j] = cw;
     rq[2 * (size_t)j + 1] = co;
+    if (dd < 0.0) {
+      const double cqt = -t * ((P.mw + P.mo) * dd);
+      qvol[2 * (size_t)j] = cqt * compi[2 * w];
+      qvol[2 * (size_t)j + 1] = cqt * compi[2 * w + 1];
+    } else {
+      qvol[2 * (size_t)j] = -t * P.mw * dd;
+      qvol[2 * (size_t)j + 1] = -t * P.mo * dd;
+    }
     dq[4 * (size_t)j + 0] = lw[0];
     dq[4 * (size_t)j + 1] = lw[1];
     dq[4 * (size_t)j + 2] = lo[0];
@@ -239,26 +277,26 @@ __global__ __launch_bounds__(64) void k_ow_wells(
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) E[a][b] = lane_sum(E[a][b], 64);
   mix_used = __any(mix_used) && !dead ? 1 : 0;
-  // Phase D: well rows
+  // Phase D: well rows (the rate rows scaled)
   for (int j = j0 + lane; j < j1; j += 64) {
     double* cp = Cp + 6 * (size_t)j;
     const double* vv = V + 4 * (size_t)j;
-    cp[0] += E[0][0] * vv[0] + E[0][1] * vv[2];
-    cp[1] += E[0][0] * vv[1] + E[0][1] * vv[3];
-    cp[2] += E[1][0] * vv[0] + E[1][1] * vv[2];
-    cp[3] += E[1][0] * vv[1] + E[1][1] * vv[3];
+    cp[0] = rs_w * (cp[0] + E[0][0] * vv[0] + E[0][1] * vv[2]);
+    cp[1] = rs_w * (cp[1] + E[0][0] * vv[1] + E[0][1] * vv[3]);
+    cp[2] = rs_o * (cp[2] + E[1][0] * vv[0] + E[1][1] * vv[2]);
+    cp[3] = rs_o * (cp[3] + E[1][0] * vv[1] + E[1][1] * vv[3]);
   }
   if (lane == 0) {
-    const bool is_bhp = wtype[w] == 0;
-    r[nc2 + 3 * (size_t)w + 0] = qw - Qw;
-    r[nc2 + 3 * (size_t)w + 1] = qo - Qo;
-    r[nc2 + 3 * (size_t)w + 2] = is_bhp ? (bh - wval[w]) : (qw + qo - wval[w]);
+    const double cw_ = wctrl[3 * w], co_ = wctrl[3 * w + 1], cb_ = wctrl[3 * w + 2];
+    r[nc2 + 3 * (size_t)w + 0] = rs_w * (qw - Qw);
+    r[nc2 + 3 * (size_t)w + 1] = rs_o * (qo - Qo);
+    r[nc2 + 3 * (size_t)w + 2] = cw_ * qw + co_ * qo + cb_ * bh - wval[w];
     double* D = Dp + 9 * (size_t)w;
-    const double loc[3][3] = {{1.0, 0.0, -dQw_b}, {0.0, 1.0, -dQo_b},
-                              {is_bhp ? 0.0 : 1.0, is_bhp ? 0.0 : 1.0, is_bhp ? 1.0 : 0.0}};
+    const double loc[3][3] = {{1.0, 0.0, -dQw_b}, {0.0, 1.0, -dQo_b}, {cw_, co_, cb_}};
+    const double rs[2] = {rs_w, rs_o};
     for (int a = 0; a < 3; ++a)
       for (int q = 0; q < 3; ++q)
-        D[a * 3 + q] = loc[a][q] + (a < 2 ? E[a][0] * F[0][q] + E[a][1] * F[1][q] : 0.0);
+        D[a * 3 + q] = a < 2 ? rs[a] * (loc[a][q] + E[a][0] * F[0][q] + E[a][1] * F[1][q]) : loc[a][q];
     mixflag[w] = mix_used;
   }
 }
@@ -282,17 +320,21 @@ __global__ void k_ow_perf_scatter(int npc, const int* __restrict__ pc_cell, cons
 }
 
 // Convergence partials: max|r_w/pv|, max|r_o/pv|, sum r_w, sum r_o,
-// sum 1/b_w(p), sum 1/b_o(p), sum p.
+// sum 1/b_w(p), sum 1/b_o(p), sum p, sum pv; with cur_pv the pore volume is
+// the current one, pv (1 + c_r (p - p_ref)) (the generic model's criterion).
 constexpr int kConvBlocks = 512;
+constexpr int kConvQ = 8;
 __global__ __launch_bounds__(256) void k_ow_conv(int nc, const double* __restrict__ r, const double* __restrict__ p,
-                                                 const double* __restrict__ pv, OWFluid f,
+                                                 const double* __restrict__ pv, OWFluid f, int cur_pv,
                                                  double* __restrict__ partial) {
-  __shared__ double sh[7][4];
+  __shared__ double sh[kConvQ][4];
   const size_t chunk = ((size_t)nc + gridDim.x - 1) / gridDim.x;
   const size_t i0 = (size_t)blockIdx.x * chunk, i1 = min((size_t)nc, i0 + chunk);
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  double v[kConvQ] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const double rw = r[2 * i], ro = r[2 * i + 1], q = pv[i];
+    const double rw = r[2 * i], ro = r[2 * i + 1];
+    const double q = (cur_pv && f.has_cr) ? pv[i] * (1.0 + f.c_r * (p[i] - f.p_ref)) : pv[i];
+    v[7] += q;
     v[0] = fmax(v[0], fabs(rw / q));
     v[1] = fmax(v[1], fabs(ro / q));
     v[2] += rw;
@@ -307,17 +349,17 @@ __global__ __launch_bounds__(256) void k_ow_conv(int nc, const double* __restric
   for (int off = 32; off > 0; off >>= 1) {
     v[0] = fmax(v[0], __shfl_xor(v[0], off, 64));
     v[1] = fmax(v[1], __shfl_xor(v[1], off, 64));
-    for (int q = 2; q < 7; ++q) v[q] += __shfl_xor(v[q], off, 64);
+    for (int q = 2; q < kConvQ; ++q) v[q] += __shfl_xor(v[q], off, 64);
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0)
-    for (int q = 0; q < 7; ++q) sh[q][wid] = v[q];
+    for (int q = 0; q < kConvQ; ++q) sh[q][wid] = v[q];
   __syncthreads();
-  if (threadIdx.x < 7) {
+  if (threadIdx.x < kConvQ) {
     const int q = threadIdx.x;
     double a = sh[q][0];
     for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a = q < 2 ? fmax(a, sh[q][k]) : a + sh[q][k];
-    partial[(size_t)blockIdx.x * 7 + q] = a;
+    partial[(size_t)blockIdx.x * kConvQ + q] = a;
   }
 }
 
@@ -325,11 +367,11 @@ __global__ __launch_bounds__(256) void k_ow_conv(int nc, const double* __restric
 // shuffle tree (fixed order, deterministic); the first 2 quantities are maxima.
 __global__ void k_ow_conv_final(int nblk, const double* __restrict__ partial, double* __restrict__ out) {
   const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  if (q >= 7) return;
+  if (q >= kConvQ) return;
   const bool is_max = q < 2;
   double a = 0.0;   // the maxima are of absolute values
   for (int b = lane; b < nblk; b += 64) {
-    const double v = partial[(size_t)b * 7 + q];
+    const double v = partial[(size_t)b * kConvQ + q];
     a = is_max ? fmax(a, v) : a + v;
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -419,7 +461,7 @@ void OWModelDev::set_topology(int nc_, int nconn_, const int* N, const double* T
   bG.upload(hG, stream);
   pv.upload(pv_host, nc, stream);
   w_cpr.alloc(2);
-  conv_partial.alloc((size_t)kConvBlocks * 7 + 8);
+  conv_partial.alloc((size_t)kConvBlocks * kConvQ + 8);
   hier_weights = hT;   // static pressure-graph strength: transmissibility
   PMX_HIP(hipStreamSynchronize(stream));
 }
@@ -436,13 +478,30 @@ void OWModelDev::set_wells(int nw_, const int* perf_ptr, const int* perf_cells, 
   WI.upload(WI_h, np, stream);
   cdp.alloc(std::max(np, 1));
   cdp.zero(stream);
-  wtype.upload(types, nw, stream);
+  // the two legacy controls: a bhp target (0) or a total surface rate (1)
+  std::vector<double> coef(3 * (size_t)nw);
+  for (int w = 0; w < nw; ++w) {
+    coef[3 * w] = coef[3 * w + 1] = types[w] == 0 ? 0.0 : 1.0;
+    coef[3 * w + 2] = types[w] == 0 ? 1.0 : 0.0;
+  }
+  wctrl.upload(coef.data(), 3 * (size_t)nw, stream);
+  wstat.upload(std::vector<int>(nw, 1).data(), nw, stream);
+  rscale[0] = rscale[1] = 1.0;
   wval.upload(vals, nw, stream);
   compi.upload(compi_h, 2 * (size_t)nw, stream);
   rq.alloc(2 * (size_t)std::max(np, 1));
+  qvol.alloc(2 * (size_t)std::max(np, 1));
   dq.alloc(4 * (size_t)std::max(np, 1));
   mixflag.alloc(std::max(nw, 1));
-  host_types.assign(types, types + nw);
+  PMX_HIP(hipStreamSynchronize(stream));
+}
+
+void OWModelDev::set_controls(const double* coef, const double* vals, const int* status, const double* rs) {
+  wctrl.upload(coef, 3 * (size_t)nw, stream);
+  wval.upload(vals, nw, stream);
+  wstat.upload(status, nw, stream);
+  rscale[0] = rs[0];
+  rscale[1] = rs[1];
   PMX_HIP(hipStreamSynchronize(stream));
 }
 
@@ -456,9 +515,9 @@ void OWModelDev::assemble(const double* p, const double* s, const double* qws, c
   PMX_HIP(hipGetLastError());
   if (nw > 0) {
     const double* c = cdp_dev ? cdp_dev : cdp.p;
-    hipLaunchKernelGGL(k_ow_wells, dim3(nw), dim3(64), 0, stream, nw, S.well_ptr.p, S.perf_cell.p, WI.p, c, wtype.p,
-                       wval.p, compi.p, p, s, qws, qos, bhp, fluid, (size_t)nc * 2, r.p, rq.p, dq.p, S.Bp.p, S.Cp.p,
-                       S.Um.p, S.V.p, S.Dp.p, mixflag.p);
+    hipLaunchKernelGGL(k_ow_wells, dim3(nw), dim3(64), 0, stream, nw, S.well_ptr.p, S.perf_cell.p, WI.p, c, wctrl.p,
+                       wstat.p, wval.p, compi.p, p, s, qws, qos, bhp, fluid, rscale[0], rscale[1], (size_t)nc * 2, r.p,
+                       rq.p, dq.p, S.Bp.p, S.Cp.p, S.Um.p, S.V.p, S.Dp.p, mixflag.p, qvol.p);
     PMX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ow_perf_scatter, dim3(grid_for(S.npc)), dim3(256), 0, stream, S.npc, S.pc_cell.p,
                        S.pc_ptr.p, S.pc_perf.p, S.diag.p, rq.p, dq.p, r.p, S.val.p);
@@ -474,31 +533,36 @@ void OWModelDev::assemble(const double* p, const double* s, const double* qws, c
 }
 
 void OWModelDev::convergence(const double* p, double* out7, double* wells_out) {
-  reduce_convergence(p, out7, wells_out, conv_partial.p, kConvBlocks);
+  reduce_convergence(p, out7, 7, wells_out, conv_partial.p, kConvBlocks, 0);
+}
+
+void OWModelDev::convergence_current_pv(const double* p, double* out8, double* wells_out) {
+  reduce_convergence(p, out8, 8, wells_out, conv_partial.p, kConvBlocks, 1);
 }
 
 // The reductions into `partial` (nblk rows of 7, then the 7 results) and
 // their copy to the host; every partial row and result is written by each
 // call, so the buffer needs no clearing between calls.
-void OWModelDev::reduce_convergence(const double* p, double* out7, double* wells_out, double* partial, int nblk) {
-  hipLaunchKernelGGL(k_ow_conv, dim3(nblk), dim3(256), 0, stream, nc, r.p, p, pv.p, fluid, partial);
+void OWModelDev::reduce_convergence(const double* p, double* out, int nout, double* wells_out, double* partial,
+                                    int nblk, int cur_pv) {
+  hipLaunchKernelGGL(k_ow_conv, dim3(nblk), dim3(256), 0, stream, nc, r.p, p, pv.p, fluid, cur_pv, partial);
   PMX_HIP(hipGetLastError());
-  double* fin = partial + (size_t)nblk * 7;
-  hipLaunchKernelGGL(k_ow_conv_final, dim3(1), dim3(64 * 7), 0, stream, nblk, partial, fin);
+  double* fin = partial + (size_t)nblk * kConvQ;
+  hipLaunchKernelGGL(k_ow_conv_final, dim3(1), dim3(64 * kConvQ), 0, stream, nblk, partial, fin);
   PMX_HIP(hipGetLastError());
-  PMX_HIP(hipMemcpyAsync(out7, fin, sizeof(double) * 7, hipMemcpyDeviceToHost, stream));
+  PMX_HIP(hipMemcpyAsync(out, fin, sizeof(double) * nout, hipMemcpyDeviceToHost, stream));
   if (nw > 0 && wells_out)
     PMX_HIP(hipMemcpyAsync(wells_out, r.p + 2 * (size_t)nc, sizeof(double) * 3 * nw, hipMemcpyDeviceToHost, stream));
   PMX_HIP(hipStreamSynchronize(stream));
 }
 
 // Allocation-free convergence reductions.  The partials live in `scratch`,
-// one copy per GPU for the life of the process: kConvBlocks rows of 7
-// doubles plus the 7 results (28,736 bytes).  Its device address is looked
+// one copy per GPU for the life of the process: kConvBlocks rows of 8
+// doubles plus the 8 results (32,832 bytes).  Its device address is looked
 // up once per GPU, with that GPU current, and only that address is handed
 // to kernels and copies.  Calls that overlap on one GPU share the array and
 // race.
-constexpr size_t kConvScratchDoubles = (size_t)kConvBlocks * 7 + 8;
+constexpr size_t kConvScratchDoubles = (size_t)kConvBlocks * kConvQ + 8;
 static __device__ __align__(256) double scratch[kConvScratchDoubles];
 constexpr int kMaxDevices = 64;
 static double* g_scratch_ptr[kMaxDevices] = {};
@@ -520,9 +584,9 @@ void OWModelDev::convergence_noalloc(int device, const double* p, double* out7, 
   PMX_CHECK(nc > 0 && r.n >= (size_t)2 * nc, "assemble before the convergence reductions");
   // one partial row per 2,048 cells, at most kConvBlocks rows
   const int nblk = (int)std::min<size_t>(kConvBlocks, ((size_t)nc + 2047) / 2048);
-  PMX_CHECK((size_t)nblk * 7 + 7 <= kConvScratchDoubles,
+  PMX_CHECK((size_t)nblk * kConvQ + kConvQ <= kConvScratchDoubles,
             "convergence reductions need more partial rows than the static scratch holds");
-  reduce_convergence(p, out7, wells_out, scratch_address(device), nblk);
+  reduce_convergence(p, out7, 7, wells_out, scratch_address(device), nblk, 0);
 }
 
 int OWModelDev::solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean) {

@@ -178,6 +178,8 @@ class Well:
     compi: np.ndarray
     sign: int = 0
     name: str = ""
+    lims: Optional[dict] = None      # limits a controlled well may switch to (types.py:42-72)
+    status: bool = True              # an open well
 
 
 def peaceman_wi(g: Grid, rock: Rock, cells, radius=0.1, skin=0.0) -> np.ndarray:
@@ -193,7 +195,7 @@ def peaceman_wi(g: Grid, rock: Rock, cells, radius=0.1, skin=0.0) -> np.ndarray:
 
 
 def add_well(wells: Optional[List[Well]], g: Grid, rock: Rock, cells, well_type="bhp", val=0.0,
-             radius=0.1, compi=None, sign=0, name=None, gravity=None) -> List[Well]:
+             radius=0.1, compi=None, sign=0, name=None, gravity=None, lims=None, status=True) -> List[Well]:
     """addWell for vertical wells (pymrst/params/wells_and_bc/add_well.py:16-126)."""
     wells = list(wells or [])
     cells = np.asarray(cells, dtype=np.int64).ravel()
@@ -206,5 +208,6 @@ def add_well(wells: Optional[List[Well]], g: Grid, rock: Rock, cells, well_type=
     wells.append(Well(cells=cells, type=well_type, val=float(val), WI=peaceman_wi(g, rock, cells, radius),
                       dZ=g.cell_centroids[cells, 2] - ref,
                       compi=np.array([1.0, 0.0, 0.0]) if compi is None else np.asarray(compi, float).ravel(),
-                      sign=int(sign), name=name or f"W{len(wells) + 1}"))
+                      sign=int(sign), name=name or f"W{len(wells) + 1}", lims=dict(lims) if lims else None,
+                      status=bool(status)))
     return wells

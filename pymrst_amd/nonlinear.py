@@ -242,6 +242,8 @@ class NonLinearSolver:
                           converged=bool(converged), residuals=dict(res), relaxation=float(relax))
             if converged:
                 report.converged = True
+                if hasattr(model, "after_convergence"):   # nonlinear_solver.py:282-288
+                    st = model.after_convergence(st, state0, dt)
                 report.wall_seconds = perf_counter() - started
                 self.convergence_issues = False
                 return st, report
