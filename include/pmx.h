@@ -272,6 +272,40 @@ int pmx_bo_solve(pmx_bo* h, const pmx_solver_options* o, double pmean, double rs
  * :439-470, _flash :472-509), in place on the device state. */
 int pmx_bo_update(pmx_bo* h, const double* dx, double ds_max, double* p, double* s, double* rs, int32_t* status,
                   double* q, double* bhp);
+/* The facility's well equations for GenericBlackOilModelAD
+ * (FacilityModel.well_equations / _single_well / _control_equation,
+ * pymrst/modules/ad_core/models/facilities/facility_model.py:2197-2292,
+ * 2434-2549, 2589-2658), held until changed, all host arrays: each well's
+ * control row c_w qWs + c_o qOs + c_g qGs + c_b bhp - val (coef[4*nw],
+ * val[nw]; the hand-rolled wells are bhp - val or qWs + qOs + qGs - val),
+ * status[nw] (1 open, 0 shut: rates zero, bhp = mean perforation
+ * pressure), and rscale[3], the rate-closure rows' per-phase scale (the
+ * facility's rhoScale, facility_model.py:2239-2263; 1 for the hand-rolled
+ * model). */
+int pmx_bo_set_controls(pmx_bo* h, const double* coef, const double* val, const int32_t* status,
+                        const double* rscale);
+/* Pressure drop from each well's bhp datum to each perforation (host,
+ * nperf, connection_pressure_drop, facility_model.py:2294-2389), held
+ * until changed (zero for the hand-rolled model, _well :383-436). */
+int pmx_bo_set_connection_drops(pmx_bo* h, const double* cdp);
+/* Reservoir volume rates of every perforation at the last pmx_bo_assemble
+ * (host, 3*nperf: a producing perforation's phase rates, an injecting
+ * one's total rate times compi; the facility's cq_vol,
+ * facility_model.py:2511-2528), which a converged step hands to the next
+ * one's wellbore column (generic_black_oil_model.py:569-601). */
+int pmx_bo_perf_volume_rates(pmx_bo* h, double* out);
+/* The Newton update rule of pmx_bo_update: 0 the hand-rolled model's
+ * (update_state :439-470), 1 update_state_black_oil_generic's
+ * (pymrst/modules/ad_blackoil/utils/update_state_black_oil_generic.py:10-104:
+ * pressure change capped at dp_max_rel |p|, rs change at drs_max, the
+ * saturation changes scaled together to at most ds_max). */
+int pmx_bo_set_update_rule(pmx_bo* h, int32_t rule, double dp_max_rel, double drs_max);
+/* Phase densities (b_w rhoS_w, b_o (rhoS_o + rs rhoS_g), b_g rhoS_g) and
+ * mobilities of every cell at a device state, into out (device, 6*nc per
+ * cell in that order): the facility's Density and Mobility at state0
+ * (facility_model.py:2223-2226). */
+int pmx_bo_cell_densities(pmx_bo* h, const double* p, const double* s, const double* rs, const int32_t* status,
+                          double* out);
 /* y = J x for the last Jacobian (device). */
 int pmx_bo_matvec(pmx_bo* h, const double* x, double* y);
 /* Block pattern (host) and 3x3 block values (device) of the last Jacobian. */

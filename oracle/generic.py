@@ -1,9 +1,12 @@
-"""GenericBlackOilModelAD in its oil-water form, with the FacilityModel well
-equations (the oracle: test infrastructure only, never the product path).
+"""GenericBlackOilModelAD with the FacilityModel well equations, in its
+oil-water form (simple fluid) and its three-phase live-oil form (deck fluid,
+disgas) (the oracle: test infrastructure only, never the product path).
 
-The generic model's reservoir equations are the hand-rolled model's (the
-reference's own proof, generic_black_oil_model.py:1026-1062, reproduces
-TwoPhaseOilWaterModelAD to 1e-11), so the cell terms are oracle/ow.py's.
+The generic model's reservoir equations are the hand-rolled models' (the
+reference's own proofs, generic_black_oil_model.py:1026-1062 and
+:1116-1190, reproduce TwoPhaseOilWaterModelAD to 1e-11 and
+ThreePhaseBlackOilModelAD on SPE 1 to 1e-11), so the cell terms are
+oracle/ow.py's and oracle/bo.py's.
 What this module restates is the facility layer and the generic model's
 step protocol:
   FacilityModel.well_equations              facility_model.py:2197-2292
@@ -28,7 +31,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .adi import ADI, ad_matmul, ad_sum, value
-from .bo import _vcat
+from .bo import BOModel, BOState, _sum, _vcat
 from .ow import OWModel, OWState
 
 RATE_TYPES = ("rate", "vrat", "resv_history")
@@ -301,3 +304,211 @@ def well_control_residual(well_residuals, control_residual, wells):
     scale = np.array([1e5 if w.type == "bhp" else (abs(w.val) if abs(w.val) > 1e-12 else max(q_ref, 1.0))
                       for w in wells])
     return float(well), float(np.max(np.abs(np.asarray(control_residual, float)) / scale))
+
+
+# ---------------------------------------------------------------------------
+# Three-phase live oil (disgas): GenericBlackOilModelAD(g, rock, deck fluid,
+# wells, disgas=True) with the facility wells.
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class GenericBOState(BOState):
+    """BOState plus the wells' perforation fluxes of the last converged
+    step (MRST's wellSol.flux), one (nperf, 3) array per well."""
+    well_flux: Optional[List[np.ndarray]] = None
+
+    def copy(self):
+        b = BOState.copy(self)
+        return GenericBOState(b.pressure, b.s, b.rs, b.status, b.qWs, b.qOs, b.qGs, b.bhp,
+                              None if self.well_flux is None else [f.copy() for f in self.well_flux])
+
+
+class GenericBOModel(BOModel):
+    """GenericBlackOilModelAD(g, rock, deck fluid, wells, disgas=True): the
+    three-phase cell equations of oracle/bo.py (the reference's own proof,
+    generic_black_oil_model.py:1116-1190, reproduces ThreePhaseBlackOilModelAD
+    on SPE 1 to 1e-11) with what differs from the hand-rolled model:
+      the facility wells (_single_well facility_model.py:2434-2549 with
+        dissolved gas carried by the oil stream :2466-2475 and the mixture's
+        volume ratio _volume_ratio :2391-2432), the wellbore column
+        (connection_pressure_drop :2294-2389) from reservoir densities with
+        dissolved gas, rhoScale from the whole grid's densities at state0
+        (:2239-2263), every control (_control_equation :2589-2658), shut wells,
+        limit switching (update_switched_controls :1103-1143);
+      update_state_black_oil_generic (update_state_black_oil_generic.py:30-99):
+        pressure change capped at dp_max_rel |p|, rs change at drs_max, the
+        saturation changes scaled together so the largest is ds_max, then the
+        flash (compute_flash_black_oil.py:22-140, oracle/bo.py's flash);
+      get_convergence (generic_black_oil_model.py:640-681): CNV/MB as the
+        hand-rolled model's (B_avg = mean(1/b), current pore volume) with the
+        wells scaled by convergence.py:72-112 and within_limits."""
+
+    def __init__(self, g, rock, fluid, wells, N, T, gravity=0.0, dp_max_rel=0.2, drs_max=np.inf,
+                 allow_cross_flow=True, **kw):
+        super().__init__(g, rock, fluid, wells, N, T, gravity=gravity, **kw)
+        self.gravity = float(gravity)
+        self.dp_max_rel, self.drs_max = float(dp_max_rel), float(drs_max)
+        self.allow_cross_flow = allow_cross_flow
+        self.within_limits = True
+        self.last_flux = None
+        self.switches = []
+        self._rho_scale = np.ones(3)
+        self._cdp = None
+        self._flux = None
+
+    def initial_state(self, *args, **kw) -> GenericBOState:
+        b = super().initial_state(*args, **kw)
+        return GenericBOState(b.pressure, b.s, b.rs, b.status, b.qWs, b.qOs, b.qGs, b.bhp, None)
+
+    # -- reservoir densities and mobilities at a state (values) -------------
+    def densities(self, st):
+        """(rho (nc, 3), mobility (nc, 3)): b_w rhoS_w, b_o (rhoS_o + rs
+        rhoS_g), b_g rhoS_g; the saturated cells' rs is rs_sat(p) and an
+        undersaturated cell has no free gas, as in the assembly."""
+        f = self.fluid
+        p = st.pressure
+        st1, st3 = st.status == 1, st.status == 3
+        rs = np.where(st1, st.rs, value(f.rs_sat(p)))
+        sw = st.s[:, 0]
+        sg = np.where(st3, st.s[:, 2], 0.0)
+        so = 1.0 - sw - sg
+        b = [value(f.b_w(p)), value(f.b_o(p, rs, st3)), value(f.b_g(p))]
+        rho = np.column_stack([b[0] * f.rhoS[0], b[1] * (f.rhoS[1] + rs * f.rhoS[2]), b[2] * f.rhoS[2]])
+        mu = [value(f.mu_w(p)), value(f.mu_o(p, rs, st3)), value(f.mu_g(p))]
+        kr = [value(f.kr_w(sw)), value(f.kr_o(sw, so, sg)), value(f.kr_g(sg))]
+        return rho, np.column_stack([kr[i] / mu[i] for i in range(3)])
+
+    def compute_cdp_per_well(self, st0):
+        """rhoScale and the column drops, both at state0 (facility_model.py:2212-2276)."""
+        rho, mob = self.densities(st0)
+        mean = rho.mean(axis=0)
+        self._rho_scale = np.array([self.fluid.rhoS[i] / mean[i] if np.isfinite(mean[i]) and mean[i] > 0 else 1.0
+                                    for i in range(3)])
+        out = []
+        for k, w in enumerate(self.wells):
+            wc = w.cells
+            flux = None if getattr(st0, "well_flux", None) is None else st0.well_flux[k]
+            out.append(column_pressure_drop(w.dZ, np.asarray(w.WI, float), rho[wc], mob[wc], w.compi, w.sign,
+                                            flux, self.gravity))
+        return out
+
+    def residual_and_jacobian(self, st0, st, dt, cdp=None):
+        self.within_limits = True
+        self._cdp = cdp if cdp is not None else [np.zeros(w.cells.size) for w in self.wells]
+        self._flux = [None] * len(self.wells)
+        out = super().residual_and_jacobian(st0, st, dt)
+        self.last_flux = list(self._flux)
+        return out
+
+    def _well(self, k, p, mob, b, rs, q_s, bhp_k):
+        """_single_well for three phases with dissolved gas."""
+        w = self.wells[k]
+        wc, tw = w.cells, np.asarray(w.WI, float)
+        compi = [float(c) for c in np.asarray(w.compi, float)[:3]]
+        dd = -(bhp_k + np.asarray(self._cdp[k], float)) + p[wc]
+        if not getattr(w, "status", True):
+            zero = 0.0 * dd
+            self._flux[k] = np.zeros((wc.size, 3))
+            ref = ad_sum(p[wc]) / max(wc.size, 1)
+            return [zero, zero, zero], [q_s[0], q_s[1], q_s[2]], bhp_k - ref
+        inj = value(dd) < 0
+        prod = ~inj
+        mob_c = [m[wc] for m in mob]
+        b_c = [x[wc] for x in b]
+        rs_c = rs[wc]
+        any_inj, any_prod = bool(inj.any()), bool(prod.any())
+        cq_p = cq_ps = None
+        mix = list(compi)
+        if any_prod:
+            con = prod * tw
+            cq_p = [-con * m * dd for m in mob_c]
+            cq_ps = [b_c[i] * cq_p[i] for i in range(3)]
+            cq_ps[2] = cq_ps[2] + rs_c * (b_c[1] * cq_p[1])
+            wbq = [q_s[i] * float(value(q_s[i])[0] > 0) - _sum(cq_ps[i]) for i in range(3)]
+            wbqt = wbq[0] + wbq[1] + wbq[2]
+            if self.allow_cross_flow and value(wbqt)[0] != 0.0:
+                mix = [q / wbqt for q in wbq]
+        if any_inj:
+            mt = mob_c[0] + mob_c[1] + mob_c[2]
+            cqt_i = -(inj * tw) * (mt * dd)
+            vol_rat = mix[0] / b_c[0] + mix[1] / b_c[1] + (mix[2] - rs_c * mix[1]) / b_c[2]
+            cqt_is = cqt_i / vol_rat
+            cq_s = [(cq_ps[i] if any_prod else 0.0) + mix[i] * cqt_is for i in range(3)]
+        else:
+            cq_s = cq_ps
+        vol = np.zeros((wc.size, 3))
+        for i in range(3):
+            if any_inj:
+                vol[:, i] += inj * (value(cqt_i) * compi[i])
+            if any_prod:
+                vol[:, i] += prod * value(cq_p[i])
+        self._flux[k] = vol
+        weqs = [(q_s[i] - _sum(cq_s[i])) * self._rho_scale[i] for i in range(3)]
+        return cq_s, weqs, control_equation(w, list(q_s), bhp_k)
+
+    # -- convergence ----------------------------------------------------------
+    def converged(self, eqs, dt, st):
+        f = self.fluid
+        res = [value(e) for e in eqs]
+        p = st.pressure
+        b = [value(f.b_w(p)), value(f.b_o(p, st.rs, st.status == 3)), value(f.b_g(p))]
+        b_avg = np.array([np.mean(1.0 / x) for x in b])
+        pv = self.pv_ref * value(f.pv_mult(p))
+        cnv = max(b_avg[i] * dt * np.abs(res[i] / pv).max() for i in range(3))
+        mb = max(b_avg[i] * dt * abs(res[i].sum()) / pv.sum() for i in range(3))
+        well_res, ctrl_res = well_control_residual(res[3:6], res[6], self.wells)
+        ok = (cnv <= self.tol_cnv and mb <= self.tol_mb and well_res <= self.tol_well and ctrl_res <= self.tol_well
+              and self.within_limits)
+        self.last_residuals = {"CNV": cnv, "MB": mb, "well": well_res, "control": ctrl_res}
+        return bool(ok), (cnv, mb)
+
+    def solve_increment(self, r, J, st):
+        import scipy.sparse.linalg as spla
+        return spla.spsolve(J.tocsc(), -r)
+
+    # -- update ----------------------------------------------------------------
+    def apply_update(self, st, upd):
+        """update_state_black_oil_generic with the flash, in place, then the
+        facility's control switching."""
+        nc, nw = self.nc, len(self.wells)
+        prev = st.copy()
+        st1, st3 = prev.status == 1, prev.status == 3
+        dp = upd[:nc]
+        cap = self.dp_max_rel * np.abs(prev.pressure)
+        st.pressure = prev.pressure + np.clip(dp, -cap, cap)
+        dsw, dx_hc = upd[nc:2 * nc], upd[2 * nc:3 * nc]
+        dsg = np.where(st3, dx_hc, 0.0)
+        drs = np.clip(np.where(st1, dx_hc, 0.0), -self.drs_max, self.drs_max)
+        st.rs = np.maximum(prev.rs + drs, 0.0)
+        ds = np.column_stack([dsw, -(dsg + dsw), dsg])
+        largest = np.abs(ds).max(axis=1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            scale = np.minimum(np.divide(self.ds_max, largest, out=np.ones_like(largest), where=largest > 0), 1.0)
+        st.s = prev.s + ds * scale[:, None]
+        off = 3 * nc
+        st.qWs = prev.qWs + upd[off:off + nw]
+        st.qOs = prev.qOs + upd[off + nw:off + 2 * nw]
+        st.qGs = prev.qGs + upd[off + 2 * nw:off + 3 * nw]
+        st.bhp = prev.bhp + upd[off + 3 * nw:off + 4 * nw]
+        self.flash(st, prev, st1)
+        self.update_switched_controls(st)
+
+    def update_switched_controls(self, st):
+        ok = True
+        for k, w in enumerate(self.wells):
+            if not getattr(w, "status", True) or not w.lims:
+                continue
+            mode = violated_limit(w, float(st.bhp[k]), np.array([st.qWs[k], st.qOs[k], st.qGs[k]]))
+            if mode is None:
+                continue
+            ok = False
+            self.switches.append((k, w.type, mode))
+            w.type = mode
+            w.val = float(w.lims[mode])
+        self.within_limits = ok
+        return ok
+
+    def after_convergence(self, st, st0, dt):
+        st.well_flux = [q.copy() for q in self.last_flux] if self.last_flux is not None else None
+        return st

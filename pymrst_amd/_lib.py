@@ -105,6 +105,11 @@ SIGNATURES = {
     "pmx_bo_convergence": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "pmx_bo_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _D, _D, _P, ctypes.POINTER(SolveInfo)]),
     "pmx_bo_update": (ctypes.c_int, [_P, _P, _D, _P, _P, _P, _P, _P, _P]),
+    "pmx_bo_set_controls": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "pmx_bo_set_connection_drops": (ctypes.c_int, [_P, _P]),
+    "pmx_bo_perf_volume_rates": (ctypes.c_int, [_P, _P]),
+    "pmx_bo_set_update_rule": (ctypes.c_int, [_P, _I, _D, _D]),
+    "pmx_bo_cell_densities": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "pmx_bo_matvec": (ctypes.c_int, [_P, _P, _P]),
     "pmx_bo_nnzb": (ctypes.c_int, [_P, ctypes.POINTER(_I)]),
     "pmx_bo_export_pattern": (ctypes.c_int, [_P, _P, _P]),

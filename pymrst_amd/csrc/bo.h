@@ -69,6 +69,16 @@ class BOModelDev {
   int solve(const LinearSolverOptions& o, double* dx, SolveInfo& info, double pmean, double rsmean);
   void update(const double* dx, double ds_max, double* p, double* s, double* rs, int* status, double* q,
               double* bhp);
+  // the facility's well pieces (GenericBlackOilModelAD's wells): per-well
+  // control rows c_w qWs + c_o qOs + c_g qGs + c_b bhp - val (coef[4 nw]),
+  // open/shut status, the rate rows' per-phase scale; per-perforation
+  // connection drops; the update rule (0 hand-rolled, 1 generic)
+  void set_controls(const double* coef, const double* val, const int* status, const double* rscale);
+  void set_connection_drops(const double* cdp_host);
+  void perf_volume_rates(double* out_host);
+  void set_update_rule(int rule, double dp_max_rel, double drs_max);
+  // per cell (rho_w, rho_o, rho_g, lambda_w, lambda_o, lambda_g) into out[6 nc] (device)
+  void cell_densities(const double* p, const double* s, const double* rs, const int* status, double* out);
 
   hipStream_t stream;
   BOFluidTables fluid;
@@ -76,8 +86,11 @@ class BOModelDev {
   BlockSystem S;
   DArr<int> bconn;
   DArr<double> bT, bG, pv, props, acc0;
-  DArr<double> WI, wval, compi, rq, dq;
-  DArr<int> wtype, mixflag;
+  DArr<double> WI, wval, compi, rq, dq, cdp, wctrl, qvol;
+  DArr<int> wtype, mixflag, wstat;
+  double rsc[3] = {1.0, 1.0, 1.0};
+  int upd_rule = 0;
+  double dp_max_rel = 0.0, drs_max = 0.0;
   DArr<double> r, rhs, w_cpr, conv_partial;
   std::vector<double> hier_weights;
   std::vector<int> line_ptr, line_cells;

@@ -442,28 +442,59 @@ __global__ __launch_bounds__(256) void k_bo_cells(int nc, const int* __restrict_
   }
 }
 
-// One 64-wide block per well (_well :383-436, no hydrostatic connection
-// drop): connection rates, well equations and the derivative pieces of the
-// well model, with the wellbore mixture of the producing connections
-// driving the injecting ones.  Duals carry (p, sW, x) of the connection
-// cell and the well's bhp.
+// One 64-wide block per well (_well :383-436; with the facility's pieces,
+// FacilityModel._single_well, facility_model.py:2434-2549): connection
+// rates, well equations and the derivative pieces of the well model, with
+// the wellbore mixture of the producing connections driving the injecting
+// ones.  Duals carry (p, sW, x) of the connection cell and the well's bhp.
+// cdp is the connection pressure drop below the bhp datum of every
+// perforation (zero for the hand-rolled model).  The control row is
+// ctrl = c_w qWs + c_o qOs + c_g qGs + c_b bhp - val (wctrl[4 w + .]); the
+// three rate-closure rows are scaled by rsc[a] (the facility's rhoScale,
+// one per phase; 1 for the hand-rolled model).  A shut well (wstat = 0)
+// takes no flux: its rows are q = 0 and bhp = the mean of its perforations'
+// pressures.  qvol receives every perforation's reservoir volume rates (a
+// producing one's phase rates, an injecting one's total times compi).
 __global__ __launch_bounds__(64) void k_bo_wells(
     int nw, const int* __restrict__ well_ptr, const int* __restrict__ perf_cell, const double* __restrict__ WI,
-    const int* __restrict__ wtype, const double* __restrict__ wval, const double* __restrict__ compi,
-    const double* __restrict__ p, const double* __restrict__ props, const double* __restrict__ q,
-    const double* __restrict__ bhp, size_t nc3, double* __restrict__ r, double* __restrict__ rq,
-    double* __restrict__ dq, double* __restrict__ Bp, double* __restrict__ Cp, double* __restrict__ Um,
-    double* __restrict__ V, double* __restrict__ Dp, int* __restrict__ mixflag) {
+    const double* __restrict__ cdp, const double* __restrict__ wctrl, const int* __restrict__ wstat,
+    const double* __restrict__ wval, const double* __restrict__ compi, const double* __restrict__ p,
+    const double* __restrict__ props, const double* __restrict__ q, const double* __restrict__ bhp, double rsc0,
+    double rsc1, double rsc2, size_t nc3, double* __restrict__ r, double* __restrict__ rq, double* __restrict__ dq,
+    double* __restrict__ Bp, double* __restrict__ Cp, double* __restrict__ Um, double* __restrict__ V,
+    double* __restrict__ Dp, int* __restrict__ mixflag, double* __restrict__ qvol) {
   const int w = blockIdx.x, lane = threadIdx.x;
   const int j0 = well_ptr[w], j1 = well_ptr[w + 1];
   const double bh = bhp[w];
   const double qs[3] = {q[3 * (size_t)w], q[3 * (size_t)w + 1], q[3 * (size_t)w + 2]};
+  const double rsc[3] = {rsc0, rsc1, rsc2};
+  if (wstat[w] == 0) {
+    const double inv_n = j1 > j0 ? 1.0 / (double)(j1 - j0) : 0.0;
+    double psum = 0.0;
+    for (int j = j0 + lane; j < j1; j += 64) {
+      psum += p[perf_cell[j]];
+      for (int a = 0; a < 3; ++a) rq[3 * (size_t)j + a] = qvol[3 * (size_t)j + a] = 0.0;
+      for (int k = 0; k < 9; ++k) dq[9 * (size_t)j + k] = Um[9 * (size_t)j + k] = V[9 * (size_t)j + k] = 0.0;
+      for (int k = 0; k < 12; ++k) Bp[12 * (size_t)j + k] = Cp[12 * (size_t)j + k] = 0.0;
+      Cp[12 * (size_t)j + 9] = -inv_n;      // d ctrl / d p of the connection cell
+    }
+    psum = lane_sum(psum, 64);
+    if (lane == 0) {
+      double* rw = r + nc3 + 4 * (size_t)w;
+      for (int a = 0; a < 3; ++a) rw[a] = qs[a];
+      rw[3] = bh - psum * inv_n;
+      double* D = Dp + 16 * (size_t)w;
+      for (int k = 0; k < 16; ++k) D[k] = (k % 5 == 0) ? 1.0 : 0.0;
+      mixflag[w] = 0;
+    }
+    return;
+  }
   // Phase A: surface rates of the producing connections and d/dbhp
   double S[3] = {0, 0, 0}, dSb[3] = {0, 0, 0};
   int anyprod = 0;
   for (int j = j0 + lane; j < j1; j += 64) {
     const int c = perf_cell[j];
-    const double dd = -bh + p[c];
+    const double dd = -(bh + cdp[j]) + p[c];
     if (dd < 0.0) continue;
     anyprod = 1;
     const BOCell C = load_cell(props + (size_t)c * kBOProps);
@@ -518,7 +549,7 @@ __global__ __launch_bounds__(64) void k_bo_wells(
     const int c = perf_cell[j];
     const BOCell C = load_cell(props + (size_t)c * kBOProps);
     const double t = WI[j];
-    const Dl<4> dd = lift<4, 0>(dvar<3>(p[c], 0)) - dvar<4>(bh, 3);
+    const Dl<4> dd = lift<4, 0>(dvar<3>(p[c], 0)) - dvar<4>(bh + cdp[j], 3);
     const Dl<4> bw = lift<4, 0>(C.bw), bo = lift<4, 0>(C.bo), bg = lift<4, 0>(C.bg), rs = lift<4, 0>(C.rs);
     Dl<4> cq[3];
     double U[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -534,6 +565,7 @@ __global__ __launch_bounds__(64) void k_bo_wells(
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) U[a][b] = (a == b ? cis.v : 0.0) - mix[a] * q2 * dvr[b];
       mix_used = 1;
+      for (int a = 0; a < 3; ++a) qvol[3 * (size_t)j + a] = cqt.v * compi[3 * (size_t)w + a];
     } else {
       const Dl<4> cw = (-t) * (lift<4, 0>(C.mw) * dd);
       const Dl<4> co = (-t) * (lift<4, 0>(C.mo) * dd);
@@ -541,6 +573,9 @@ __global__ __launch_bounds__(64) void k_bo_wells(
       cq[0] = bw * cw;
       cq[1] = bo * co;
       cq[2] = bg * cg + rs * (bo * co);
+      qvol[3 * (size_t)j] = cw.v;
+      qvol[3 * (size_t)j + 1] = co.v;
+      qvol[3 * (size_t)j + 2] = cg.v;
       // d mix / d (p, sW, x)_c through wbq = ... - sum cq_ps
       for (int a = 0; a < 3; ++a)
         for (int k = 0; k < 3; ++k)
@@ -578,19 +613,19 @@ __global__ __launch_bounds__(64) void k_bo_wells(
     const double* vv = V + 9 * (size_t)j;
     for (int a = 0; a < 3; ++a)
       for (int k = 0; k < 3; ++k)
-        cp[3 * a + k] += E[a][0] * vv[k] + E[a][1] * vv[3 + k] + E[a][2] * vv[6 + k];
+        cp[3 * a + k] = rsc[a] * (cp[3 * a + k] + E[a][0] * vv[k] + E[a][1] * vv[3 + k] + E[a][2] * vv[6 + k]);
   }
   if (lane == 0) {
-    const bool is_bhp = wtype[w] == 0;
+    const double* cc = wctrl + 4 * (size_t)w;
     double* rw = r + nc3 + 4 * (size_t)w;
-    for (int a = 0; a < 3; ++a) rw[a] = qs[a] - Q[a];
-    rw[3] = is_bhp ? (bh - wval[w]) : (qs[0] + qs[1] + qs[2] - wval[w]);
+    for (int a = 0; a < 3; ++a) rw[a] = rsc[a] * (qs[a] - Q[a]);
+    rw[3] = cc[0] * qs[0] + cc[1] * qs[1] + cc[2] * qs[2] + cc[3] * bh - wval[w];
     double* D = Dp + 16 * (size_t)w;
     for (int a = 0; a < 3; ++a)
       for (int qq = 0; qq < 4; ++qq)
-        D[4 * a + qq] = (a == qq ? 1.0 : 0.0) + (qq == 3 ? -dQb[a] : 0.0) + E[a][0] * Fm[0][qq] +
-                        E[a][1] * Fm[1][qq] + E[a][2] * Fm[2][qq];
-    for (int qq = 0; qq < 4; ++qq) D[12 + qq] = is_bhp ? (qq == 3 ? 1.0 : 0.0) : (qq == 3 ? 0.0 : 1.0);
+        D[4 * a + qq] = rsc[a] * ((a == qq ? 1.0 : 0.0) + (qq == 3 ? -dQb[a] : 0.0) + E[a][0] * Fm[0][qq] +
+                                  E[a][1] * Fm[1][qq] + E[a][2] * Fm[2][qq]);
+    for (int qq = 0; qq < 4; ++qq) D[12 + qq] = cc[qq];
     mixflag[w] = mix_used;
   }
 }
@@ -675,9 +710,16 @@ __global__ void k_bo_conv_final(int nblk, const double* __restrict__ partial, do
 
 // Newton update with the variable switch and the flash (update_state
 // :439-470, _flash :472-509), one thread per cell; well unknowns after.
-__global__ void k_bo_update(int nc, int nw, const double* __restrict__ dx, double ds_max, BOFluidView F,
-                            double* __restrict__ p, double* __restrict__ s, double* __restrict__ rs,
-                            int* __restrict__ status, double* __restrict__ q, double* __restrict__ bhp) {
+// rule 0 is the hand-rolled model's: each saturation change clipped to
+// +-ds_max, pressure unlimited.  rule 1 is update_state_black_oil_generic's
+// (update_state_black_oil_generic.py:30-80): the pressure change capped at
+// dp_max_rel |p|, the rs change at drs_max, and the three saturation changes
+// scaled together so that the largest is at most ds_max; the flash is the
+// same (compute_flash_black_oil.py:22-140 without vaporised oil).
+__global__ void k_bo_update(int nc, int nw, const double* __restrict__ dx, double ds_max, int rule,
+                            double dp_max_rel, double drs_max, BOFluidView F, double* __restrict__ p,
+                            double* __restrict__ s, double* __restrict__ rs, int* __restrict__ status,
+                            double* __restrict__ q, double* __restrict__ bhp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nc) {
     const double etol = 1.4901161193847656e-08;   // sqrt(DBL_EPSILON)
@@ -685,13 +727,25 @@ __global__ void k_bo_update(int nc, int nw, const double* __restrict__ dx, doubl
     const double p_prev = p[i], rs_prev = rs[i];
     const double sw_prev = s[3 * (size_t)i], so_prev = s[3 * (size_t)i + 1], sg_prev = s[3 * (size_t)i + 2];
     const double dp = dx[3 * (size_t)i], dsw = dx[3 * (size_t)i + 1], dxx = dx[3 * (size_t)i + 2];
-    const double pn = p_prev + dp;
     const double dsg = st3 ? dxx : 0.0;
-    double r = fmax(rs_prev + (st1 ? dxx : 0.0), 0.0);
     const double ds[3] = {dsw, -(dsg + dsw), dsg};
-    double sw = sw_prev + fmin(fmax(ds[0], -ds_max), ds_max);
-    double so = so_prev + fmin(fmax(ds[1], -ds_max), ds_max);
-    double sg = sg_prev + fmin(fmax(ds[2], -ds_max), ds_max);
+    double pn, r, sw, so, sg;
+    if (rule == 0) {
+      pn = p_prev + dp;
+      r = fmax(rs_prev + (st1 ? dxx : 0.0), 0.0);
+      sw = sw_prev + fmin(fmax(ds[0], -ds_max), ds_max);
+      so = so_prev + fmin(fmax(ds[1], -ds_max), ds_max);
+      sg = sg_prev + fmin(fmax(ds[2], -ds_max), ds_max);
+    } else {
+      const double cap = dp_max_rel * fabs(p_prev);
+      pn = p_prev + fmin(fmax(dp, -cap), cap);
+      r = fmax(rs_prev + fmin(fmax(st1 ? dxx : 0.0, -drs_max), drs_max), 0.0);
+      const double largest = fmax(fmax(fabs(ds[0]), fabs(ds[1])), fabs(ds[2]));
+      const double sc = largest > 0.0 ? fmin(ds_max / largest, 1.0) : 1.0;
+      sw = sw_prev + ds[0] * sc;
+      so = so_prev + ds[1] * sc;
+      sg = sg_prev + ds[2] * sc;
+    }
     // flash
     const bool wat_only = sw > 1.0 - etol;
     double d;
@@ -941,8 +995,73 @@ void BOModelDev::set_wells(int nw_, const int* perf_ptr, const int* perf_cells, 
   compi.upload(compi_h, 3 * (size_t)nw, stream);
   rq.alloc(3 * (size_t)std::max(np, 1));
   dq.alloc(9 * (size_t)std::max(np, 1));
+  qvol.alloc(3 * (size_t)std::max(np, 1));
   mixflag.alloc(std::max(nw, 1));
+  // the hand-rolled controls: bhp - val, or qWs + qOs + qGs - val
+  std::vector<double> coef(4 * (size_t)std::max(nw, 1), 0.0);
+  for (int k = 0; k < nw; ++k) {
+    if (types[k] == 0) coef[4 * k + 3] = 1.0;
+    else coef[4 * k] = coef[4 * k + 1] = coef[4 * k + 2] = 1.0;
+  }
+  wctrl.upload(coef.data(), coef.size(), stream);
+  wstat.upload(std::vector<int>(std::max(nw, 1), 1).data(), std::max(nw, 1), stream);
+  cdp.upload(std::vector<double>(std::max(np, 1), 0.0).data(), std::max(np, 1), stream);
   PMX_HIP(hipStreamSynchronize(stream));
+}
+
+void BOModelDev::set_controls(const double* coef, const double* val, const int* status, const double* rscale) {
+  if (nw > 0) {
+    wctrl.upload(coef, 4 * (size_t)nw, stream);
+    wval.upload(val, nw, stream);
+    wstat.upload(status, nw, stream);
+  }
+  for (int a = 0; a < 3; ++a) rsc[a] = rscale ? rscale[a] : 1.0;
+  PMX_HIP(hipStreamSynchronize(stream));
+}
+
+void BOModelDev::set_connection_drops(const double* cdp_host) {
+  const int np = nw ? S.nperf : 0;
+  if (np > 0) cdp.upload(cdp_host, np, stream);
+  PMX_HIP(hipStreamSynchronize(stream));
+}
+
+void BOModelDev::perf_volume_rates(double* out_host) {
+  const int np = nw ? S.nperf : 0;
+  if (np > 0) PMX_HIP(hipMemcpyAsync(out_host, qvol.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, stream));
+  PMX_HIP(hipStreamSynchronize(stream));
+}
+
+void BOModelDev::set_update_rule(int rule, double dpmr, double drsm) {
+  PMX_CHECK(rule == 0 || rule == 1, "update rule is 0 (hand-rolled) or 1 (generic)");
+  upd_rule = rule;
+  dp_max_rel = dpmr;
+  drs_max = drsm;
+}
+
+// Phase densities (water b_w rhoS_w, oil b_o (rhoS_o + rs rhoS_g), gas
+// b_g rhoS_g) and mobilities of every cell at a state.
+__global__ __launch_bounds__(256) void k_bo_densities(int nc, const double* __restrict__ p,
+                                                      const double* __restrict__ s, const double* __restrict__ rs,
+                                                      const int* __restrict__ status, BOFluidView F,
+                                                      double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const BOCell c = bo_cell(F, p[i], s[3 * (size_t)i], s[3 * (size_t)i + 2], rs[i], status[i]);
+  double* o = out + 6 * (size_t)i;
+  o[0] = c.bw.v * F.rhoW;
+  o[1] = c.bo.v * (F.rhoO + c.rs.v * F.rhoG);
+  o[2] = c.bg.v * F.rhoG;
+  o[3] = c.mw.v;
+  o[4] = c.mo.v;
+  o[5] = c.mg.v;
+}
+
+void BOModelDev::cell_densities(const double* p, const double* s, const double* rs, const int* status,
+                                double* out) {
+  PMX_CHECK(nc > 0, "topology not set");
+  hipLaunchKernelGGL(k_bo_densities, dim3(grid_for(nc)), dim3(256), 0, stream, nc, p, s, rs, status, fluid.view(),
+                     out);
+  PMX_HIP(hipGetLastError());
 }
 
 void BOModelDev::begin_step(const double* p0, const double* s0, const double* rs0, const int* status0) {
@@ -965,9 +1084,9 @@ void BOModelDev::assemble(const double* p, const double* s, const double* rs, co
                      S.val.p);
   PMX_HIP(hipGetLastError());
   if (nw > 0) {
-    hipLaunchKernelGGL(k_bo_wells, dim3(nw), dim3(64), 0, stream, nw, S.well_ptr.p, S.perf_cell.p, WI.p, wtype.p,
-                       wval.p, compi.p, p, props.p, q, bhp, (size_t)nc * 3, r.p, rq.p, dq.p, S.Bp.p, S.Cp.p, S.Um.p,
-                       S.V.p, S.Dp.p, mixflag.p);
+    hipLaunchKernelGGL(k_bo_wells, dim3(nw), dim3(64), 0, stream, nw, S.well_ptr.p, S.perf_cell.p, WI.p, cdp.p,
+                       wctrl.p, wstat.p, wval.p, compi.p, p, props.p, q, bhp, rsc[0], rsc[1], rsc[2], (size_t)nc * 3,
+                       r.p, rq.p, dq.p, S.Bp.p, S.Cp.p, S.Um.p, S.V.p, S.Dp.p, mixflag.p, qvol.p);
     PMX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bo_perf_scatter, dim3(grid_for(S.npc)), dim3(256), 0, stream, S.npc, S.pc_cell.p,
                        S.pc_ptr.p, S.pc_perf.p, S.diag.p, rq.p, dq.p, r.p, S.val.p);
@@ -1026,7 +1145,7 @@ int BOModelDev::solve(const LinearSolverOptions& o, double* dx, SolveInfo& info,
 void BOModelDev::update(const double* dx, double ds_max, double* p, double* s, double* rs, int* status, double* q,
                         double* bhp) {
   hipLaunchKernelGGL(k_bo_update, dim3(grid_for((size_t)nc + nw)), dim3(256), 0, stream, nc, nw, dx, ds_max,
-                     fluid.view(), p, s, rs, status, q, bhp);
+                     upd_rule, dp_max_rel, drs_max, fluid.view(), p, s, rs, status, q, bhp);
   PMX_HIP(hipGetLastError());
 }
 

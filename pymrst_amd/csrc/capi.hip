@@ -740,6 +740,50 @@ int pmx_bo_update(pmx_bo* h, const double* dx, double ds_max, double* p, double*
   })
 }
 
+int pmx_bo_set_controls(pmx_bo* h, const double* coef, const double* val, const int32_t* status,
+                        const double* rscale) {
+  PMX_TRY({
+    PMX_CHECK(h && rscale, "null argument");
+    PMX_CHECK(h->m->nw == 0 || (coef && val && status), "well controls required");
+    for (int k = 0; k < h->m->nw; ++k) PMX_CHECK(status[k] == 0 || status[k] == 1, "well status is 0 or 1");
+    bind(h);
+    h->m->set_controls(coef, val, status, rscale);
+  })
+}
+
+int pmx_bo_set_connection_drops(pmx_bo* h, const double* cdp) {
+  PMX_TRY({
+    PMX_CHECK(h && (h->m->nw == 0 || cdp), "null argument");
+    bind(h);
+    h->m->set_connection_drops(cdp);
+  })
+}
+
+int pmx_bo_perf_volume_rates(pmx_bo* h, double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && (h->m->nw == 0 || out), "null argument");
+    bind(h);
+    h->m->perf_volume_rates(out);
+  })
+}
+
+int pmx_bo_set_update_rule(pmx_bo* h, int32_t rule, double dp_max_rel, double drs_max) {
+  PMX_TRY({
+    PMX_CHECK(h, "null argument");
+    PMX_CHECK(dp_max_rel >= 0.0 && drs_max >= 0.0, "update limits must be non-negative");
+    h->m->set_update_rule(rule, dp_max_rel, drs_max);
+  })
+}
+
+int pmx_bo_cell_densities(pmx_bo* h, const double* p, const double* s, const double* rs, const int32_t* status,
+                          double* out) {
+  PMX_TRY({
+    PMX_CHECK(h && p && s && rs && status && out, "null argument");
+    bind(h);
+    h->m->cell_densities(p, s, rs, status, out);
+  })
+}
+
 int pmx_bo_matvec(pmx_bo* h, const double* x, double* y) {
   PMX_TRY({
     PMX_CHECK(h && x && y, "null argument");

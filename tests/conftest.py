@@ -1,7 +1,13 @@
 import os
 import sys
 
-import pytest
+# The oracle's sparse and small dense algebra runs fastest on one BLAS
+# thread, and parallel test workers (pytest -n) each spinning a full
+# thread pool slow one another down by orders of magnitude.
+for _var in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ.setdefault(_var, "1")
+
+import pytest  # noqa: E402
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
