@@ -316,6 +316,38 @@ int pmx_bo_export_blocks(pmx_bo* h, double* vals);
 int pmx_bo_profile(pmx_bo* h, int32_t enable);
 int pmx_bo_profile_read(pmx_bo* h, double* out4);
 
+/* ---- corner-point grids: MRST's processGRDECL topology
+ *      (pymrst/gridprocessing/process_grdecl.py:352-580) on the host and
+ *      computeGeometry (pymrst/gridprocessing/compute_geometry.py:211-325)
+ *      on the device */
+typedef struct pmx_cp pmx_cp;
+
+/* Process a corner-point deck (host arrays): coord[6*(nx+1)*(ny+1)] pillar
+ * end points (COORD), zcorn[8*nx*ny*nz] corner depths (ZCORN, in MRST's
+ * order), actnum[nx*ny*nz] (ACTNUM; null: all active).  Faults (pillar
+ * pairs whose cell stacks do not match) are split into the overlaps of the
+ * cells on either side; zero-thickness layers join the cells above and
+ * below them; inactive and pinched cells are dropped. */
+int pmx_cp_process(int32_t nx, int32_t ny, int32_t nz, const double* coord, const double* zcorn,
+                   const int32_t* actnum, pmx_cp** out);
+int pmx_cp_destroy(pmx_cp* h);
+/* out4 = {cells, faces, nodes, face-node entries}. */
+int pmx_cp_sizes(pmx_cp* h, int32_t* out4);
+/* The grid (host arrays, sized by pmx_cp_sizes): index_map[nc] (Cartesian
+ * index i + nx (j + ny k) of each cell), neighbors[2*nf] (-1 outside; a
+ * face's area vector points from its first cell to its second), tag[nf]
+ * (1 i-face, 2 j-face, 3 k-face; +3 on a fault), node_pos[nf+1] and
+ * face_nodes[entries] (faces.nodePos / faces.nodes), coords[3*nn],
+ * face_pos[nc+1] and cell_faces[2*internal+boundary] (cells.facePos /
+ * cells.faces), cell_dims[3*nc] (extents of each cell's corner points). */
+int pmx_cp_topology(pmx_cp* h, int32_t* index_map, int32_t* neighbors, int32_t* tag, int32_t* node_pos,
+                    int32_t* face_nodes, double* coords, int32_t* face_pos, int32_t* cell_faces, double* cell_dims);
+/* computeGeometry on the device (device arrays): face_normals[3*nf] (area
+ * vectors), face_areas[nf], face_centroids[3*nf], cell_volumes[nc],
+ * cell_centroids[3*nc]. */
+int pmx_cp_geometry(pmx_cp* h, int32_t device, void* hip_stream, double* face_normals, double* face_areas,
+                    double* face_centroids, double* cell_volumes, double* cell_centroids);
+
 /* ---- incompressible TPFA pressure solve (incomp_tpfa_ad,
  *      pymrst/modules/incomp/incomp_tpfa.py:252-389) and computeTrans
  *      (pymrst/solvers/compute_trans.py:11-29) */

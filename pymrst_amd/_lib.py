@@ -116,6 +116,11 @@ SIGNATURES = {
     "pmx_bo_export_blocks": (ctypes.c_int, [_P, _P]),
     "pmx_bo_profile": (ctypes.c_int, [_P, _I]),
     "pmx_bo_profile_read": (ctypes.c_int, [_P, _P]),
+    "pmx_cp_process": (ctypes.c_int, [_I, _I, _I, _P, _P, _P, ctypes.POINTER(_P)]),
+    "pmx_cp_destroy": (ctypes.c_int, [_P]),
+    "pmx_cp_sizes": (ctypes.c_int, [_P, _P]),
+    "pmx_cp_topology": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "pmx_cp_geometry": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "pmx_incomp_create": (ctypes.c_int, [_I, _P, ctypes.POINTER(_P)]),
     "pmx_incomp_destroy": (ctypes.c_int, [_P]),
     "pmx_incomp_set_grid": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P]),

@@ -15,7 +15,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "_pmx.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["topology.cpp", "linsys.hip", "ow.hip", "bo.hip", "adjoint.hip", "incomp.hip", "capi.hip"]
+SOURCES = ["topology.cpp", "linsys.hip", "ow.hip", "bo.hip", "adjoint.hip", "incomp.hip", "cpgrid.hip",
+           "capi.hip"]
 FLAGS = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-command-line-argument"]

@@ -10,6 +10,7 @@
 #include "incomp.h"
 #include "ow.h"
 #include "bo.h"
+#include "cpgrid.h"
 
 using namespace pmx;
 
@@ -22,6 +23,10 @@ struct pmx_ow {
   int device = 0;
   std::unique_ptr<OWModelDev> m;
   pmx_solver_options last_opts{};
+};
+
+struct pmx_cp {
+  CpGrid g;
 };
 
 struct pmx_bo {
@@ -986,6 +991,62 @@ int pmx_sys_solve(pmx_sys* h, const pmx_solver_options* opts, const double* w, c
     return -1;
   }
   return status;
+}
+
+
+int pmx_cp_process(int32_t nx, int32_t ny, int32_t nz, const double* coord, const double* zcorn,
+                   const int32_t* actnum, pmx_cp** out) {
+  PMX_TRY({
+    PMX_CHECK(out, "null argument");
+    *out = nullptr;
+    std::unique_ptr<pmx_cp> h(new pmx_cp());
+    cp_process(nx, ny, nz, coord, zcorn, actnum, h->g);
+    *out = h.release();
+  })
+}
+
+int pmx_cp_destroy(pmx_cp* h) {
+  delete h;
+  return 0;
+}
+
+int pmx_cp_sizes(pmx_cp* h, int32_t* out4) {
+  PMX_TRY({
+    PMX_CHECK(h && out4, "null argument");
+    out4[0] = h->g.nc;
+    out4[1] = h->g.nf;
+    out4[2] = h->g.nn;
+    out4[3] = (int32_t)h->g.face_nodes.size();
+  })
+}
+
+int pmx_cp_topology(pmx_cp* h, int32_t* index_map, int32_t* neighbors, int32_t* tag, int32_t* node_pos,
+                    int32_t* face_nodes, double* coords, int32_t* face_pos, int32_t* cell_faces, double* cell_dims) {
+  PMX_TRY({
+    PMX_CHECK(h && index_map && neighbors && tag && node_pos && face_nodes && coords && face_pos && cell_faces &&
+                  cell_dims,
+              "null argument");
+    const CpGrid& g = h->g;
+    std::copy(g.index_map.begin(), g.index_map.end(), index_map);
+    std::copy(g.nb.begin(), g.nb.end(), neighbors);
+    std::copy(g.tag.begin(), g.tag.end(), tag);
+    std::copy(g.node_pos.begin(), g.node_pos.end(), node_pos);
+    std::copy(g.face_nodes.begin(), g.face_nodes.end(), face_nodes);
+    std::copy(g.coords.begin(), g.coords.end(), coords);
+    std::copy(g.face_pos.begin(), g.face_pos.end(), face_pos);
+    std::copy(g.cell_faces.begin(), g.cell_faces.end(), cell_faces);
+    std::copy(g.cell_dims.begin(), g.cell_dims.end(), cell_dims);
+  })
+}
+
+int pmx_cp_geometry(pmx_cp* h, int32_t device, void* hip_stream, double* face_normals, double* face_areas,
+                    double* face_centroids, double* cell_volumes, double* cell_centroids) {
+  PMX_TRY({
+    PMX_CHECK(h && face_normals && face_areas && face_centroids && cell_volumes && cell_centroids, "null argument");
+    PMX_HIP(hipSetDevice(device));
+    cp_geometry(h->g, (hipStream_t)hip_stream, face_normals, face_areas, face_centroids, cell_volumes,
+                cell_centroids);
+  })
 }
 
 }  // extern "C"

@@ -16,6 +16,10 @@ Sources (relative to the reference checkout):
 * examples/validation/ref_ad_core_timesteps.mat - MRST's time-step selectors
   driven through eight scripted ministeps, and rampupTimesteps (script
   examples/validation/dump_ad_core_timesteps_reference.m)
+* examples/validation/ref_grid_process_grdecl_omo.mat - MRST processGRDECL
+  + computeGeometry on a 3x2x2 unit-cube corner-point deck with one inactive
+  cell (deck in examples/validation/validate_grid_process_grdecl_omo.py):
+  cell, face and node counts, indexMap, sorted centroids and volumes
 
 Run:  python tests/golden/make_fixtures.py [/path/to/reference]
 """
@@ -43,6 +47,10 @@ def main(ref_root="/root/reference"):
     np.savez_compressed(HERE / "bo_phase_flux.npz",
                         **_mat(val / "ref_ad_blackoil_phase_flux.mat"))
     np.savez_compressed(HERE / "ad_core_timesteps.npz", **_mat(val / "ref_ad_core_timesteps.mat"))
+    cp = _mat(val / "ref_grid_process_grdecl_omo.mat")
+    np.savez_compressed(HERE / "cp_grdecl_omo.npz", **{k: cp[k] for k in (
+        "cell_count", "face_count", "node_count", "index_map", "sorted_centroids", "sorted_volumes",
+        "total_volume")})
     pw = np.load(ref / "paired_waterflood_pymrst_results.npz", allow_pickle=False)
     np.savez_compressed(HERE / "paired_waterflood.npz",
                         **{k: np.asarray(pw[k]) for k in pw.files})
