@@ -31,8 +31,12 @@ HBM_PEAK_GBS = 8000.0
 # calibrated against a 512 MiB copy (fetch factor 2.0, write factor 1.0):
 # 404.3 MB read + 17.95 MB written.
 MEASURED_SPMV_TRAFFIC = {(60, 220, 85): 422.2e6}
-# k_line_gs_tw<2,float,8> per launch (one colour)
-MEASURED_LINE_GS_TRAFFIC = {}
+# k_line_gs_tw<2,float,8> per launch (one colour), mean over the ten launches
+# of one preconditioner application, from the same counter runs over the
+# first report step of the 60x220x85 run: 32.87 MB read + 8.92 MB written in
+# each of the eight launches of the first four sweeps, 40.82 + 24.63 MB in
+# each of the two of the last sweep.
+MEASURED_LINE_GS_TRAFFIC = {(60, 220, 85): 46.5e6}
 
 
 def spmv_bytes(nnzb: int, nc: int) -> float:
