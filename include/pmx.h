@@ -191,6 +191,18 @@ int pmx_ow_set_segments(pmx_ow* h, int32_t nseg, const int32_t* cell_ptr, const 
  * zero rows of members that have converged, so their Newton update is zero
  * while the others iterate.  No reference counterpart. */
 int pmx_ow_mask_residual(pmx_ow* h, const double* mask);
+/* Members of a union model (an ensemble): member m owns the cells
+ * [cell_ptr[m], cell_ptr[m+1]) (host array of nmem+1, ascending, from 0 to
+ * the model's cell count).  Set once per model; pmx_ow_member_convergence
+ * then reduces each member's cells on its own.  No reference counterpart:
+ * the reference loops over realisations. */
+int pmx_ow_set_members(pmx_ow* h, int32_t nmem, const int32_t* cell_ptr);
+/* pmx_ow_convergence per member: the quantities of the reference's
+ * _converged (two_phase_oil_water_model.py:549-567) for each member's own
+ * cells, out7 (host, nmem rows of 7) = {max|r_w/pv|, max|r_o/pv|, sum r_w,
+ * sum r_o, sum 1/b_w(p), sum 1/b_o(p), sum p}, reduced in a fixed order;
+ * wells_out (host, 3 nw, or NULL) = the well residual rows. */
+int pmx_ow_member_convergence(pmx_ow* h, const double* p, double* out7, double* wells_out);
 /* Per-cell step lengths for the accumulation terms of the following
  * pmx_ow_assemble calls (device array of nc values, caller-owned, read at
  * every assembly), so that ensemble members advance with their own time

@@ -90,6 +90,8 @@ SIGNATURES = {
     "pmx_ow_set_segments": (ctypes.c_int, [_P, _I, _P, _P]),
     "pmx_ow_mask_residual": (ctypes.c_int, [_P, _P]),
     "pmx_ow_set_cell_dt": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_set_members": (ctypes.c_int, [_P, _I, _P]),
+    "pmx_ow_member_convergence": (ctypes.c_int, [_P, _P, _P, _P]),
     "pmx_ow_adjoint_solve": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), ctypes.c_double, _P, _P,
                                             ctypes.POINTER(SolveInfo)]),
     "pmx_ow_prev_state_transpose": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, _P, _P]),

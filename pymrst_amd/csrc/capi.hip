@@ -614,6 +614,22 @@ int pmx_ow_set_cell_dt(pmx_ow* h, const double* dt_cell) {
   })
 }
 
+int pmx_ow_set_members(pmx_ow* h, int32_t nmem, const int32_t* cell_ptr) {
+  PMX_TRY({
+    PMX_CHECK(h && cell_ptr && nmem >= 1, "bad argument");
+    bind(h);
+    h->m->set_members(nmem, cell_ptr);
+  })
+}
+
+int pmx_ow_member_convergence(pmx_ow* h, const double* p, double* out7, double* wells_out) {
+  PMX_TRY({
+    PMX_CHECK(h && p && out7, "null argument");
+    bind(h);
+    h->m->member_convergence(p, out7, wells_out);
+  })
+}
+
 int pmx_ow_mask_residual(pmx_ow* h, const double* mask) {
   PMX_TRY({
     PMX_CHECK(h && mask, "null argument");

@@ -36,6 +36,12 @@ class OWModelDev {
   void update(const double* dx, double sign, double ds_max, double dp_max_rel, double* p, double* s, double* qws,
               double* qos, double* bhp);
   void mask_residual(const double* mask);   // r *= mask (device, per unknown)
+  // members of a union model: consecutive cell ranges [cell_ptr[m],
+  // cell_ptr[m+1]); member_convergence reduces each member's cells (out:
+  // nmem rows of max|r_w/pv|, max|r_o/pv|, sum r_w, sum r_o, sum 1/b_w(p),
+  // sum 1/b_o(p), sum p), in a fixed order
+  void set_members(int nmem, const int* cell_ptr);
+  void member_convergence(const double* p, double* out7, double* wells_out);
   const double* dt_cell = nullptr;          // per-cell step length (device, caller-owned), or the scalar dt
 
   // adjoint (adjoint.hip): transposed solve of the last assembled Jacobian,
@@ -57,6 +63,9 @@ class OWModelDev {
   DArr<int> wstat, mixflag;
   double rscale[2] = {1.0, 1.0};
   DArr<double> r, rhs, w_cpr, conv_partial;
+  int nmem = 0, nchunk = 0;
+  DArr<int> mem_chunk, mem_chunk_ptr;      // per chunk (i0, i1); chunks of member m: [ptr[m], ptr[m+1])
+  DArr<double> mem_partial, mem_out;
   std::vector<double> hier_weights;
   std::vector<int> host_bconn, line_ptr, line_cells;
   CprSolver* cpr = nullptr;

@@ -381,6 +381,71 @@ __global__ void k_ow_conv_final(int nblk, const double* __restrict__ partial, do
   if (lane == 0) out[q] = a;
 }
 
+// Member-wise convergence partials of a union model: one workgroup per
+// chunk of at most kMemChunk cells of one member (a chunk never holds two
+// members' cells), the seven quantities of k_ow_conv other than the
+// pore-volume sum.  Every partial row is written by each launch.
+constexpr int kMemQ = 7;
+constexpr int kMemChunk = 2048;
+__global__ __launch_bounds__(256) void k_ow_conv_chunks(const int* __restrict__ chunk, const double* __restrict__ r,
+                                                        const double* __restrict__ p, const double* __restrict__ pv,
+                                                        OWFluid f, double* __restrict__ partial) {
+  __shared__ double sh[kMemQ][4];
+  const int i0 = chunk[2 * blockIdx.x], i1 = chunk[2 * blockIdx.x + 1];
+  double v[kMemQ] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += (int)blockDim.x) {
+    const double rw = r[2 * (size_t)i], ro = r[2 * (size_t)i + 1], q = pv[i];
+    v[0] = fmax(v[0], fabs(rw / q));
+    v[1] = fmax(v[1], fabs(ro / q));
+    v[2] += rw;
+    v[3] += ro;
+    double b, db;
+    shrink(f.c_w, p[i], f.p_ref, b, db);
+    v[4] += 1.0 / b;
+    shrink(f.c_o, p[i], f.p_ref, b, db);
+    v[5] += 1.0 / b;
+    v[6] += p[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    v[0] = fmax(v[0], __shfl_xor(v[0], off, 64));
+    v[1] = fmax(v[1], __shfl_xor(v[1], off, 64));
+    for (int q = 2; q < kMemQ; ++q) v[q] += __shfl_xor(v[q], off, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < kMemQ; ++q) sh[q][wid] = v[q];
+  __syncthreads();
+  if (threadIdx.x < kMemQ) {
+    const int q = threadIdx.x;
+    double a = sh[q][0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) a = q < 2 ? fmax(a, sh[q][k]) : a + sh[q][k];
+    partial[(size_t)blockIdx.x * kMemQ + q] = a;
+  }
+}
+
+// One wave per member: lanes stride over the member's chunk partials, then a
+// fixed shuffle tree per quantity (deterministic).  A member without cells
+// gets zeros.
+__global__ __launch_bounds__(64) void k_ow_conv_members(const int* __restrict__ ptr,
+                                                        const double* __restrict__ partial,
+                                                        double* __restrict__ out) {
+  const int m = blockIdx.x, lane = threadIdx.x;
+  const int c0 = ptr[m], c1 = ptr[m + 1];
+  for (int q = 0; q < kMemQ; ++q) {
+    const bool is_max = q < 2;
+    double a = 0.0;
+    for (int c = c0 + lane; c < c1; c += 64) {
+      const double v = partial[(size_t)c * kMemQ + q];
+      a = is_max ? fmax(a, v) : a + v;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(a, off, 64);
+      a = is_max ? fmax(a, o) : a + o;
+    }
+    if (lane == 0) out[(size_t)m * kMemQ + q] = a;
+  }
+}
+
 // Newton update with the reference's limits (step :629-651): dp clipped to
 // +-dp_max_rel |p|, dsW to +-ds_max, sW to [0, 1]; dx holds sign * update.
 __global__ void k_ow_update(int nc, int nw, const double* __restrict__ dx, double sign, double ds_max,
@@ -611,6 +676,43 @@ int OWModelDev::solve(const LinearSolverOptions& o, double* dx, SolveInfo& info,
 __global__ void k_mask(size_t n, const double* __restrict__ mask, double* __restrict__ r) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) r[i] *= mask[i];
+}
+
+void OWModelDev::set_members(int nm, const int* cell_ptr) {
+  PMX_CHECK(nm >= 1 && cell_ptr[0] == 0 && cell_ptr[nm] == nc, "member cell ranges must cover the cells");
+  std::vector<int> chunk, ptr(1, 0);
+  for (int m = 0; m < nm; ++m) {
+    PMX_CHECK(cell_ptr[m + 1] >= cell_ptr[m], "member cell ranges must be ascending");
+    for (int i0 = cell_ptr[m]; i0 < cell_ptr[m + 1]; i0 += kMemChunk) {
+      chunk.push_back(i0);
+      chunk.push_back(std::min(i0 + kMemChunk, cell_ptr[m + 1]));
+    }
+    ptr.push_back((int)chunk.size() / 2);
+  }
+  nmem = nm;
+  nchunk = ptr.back();
+  mem_chunk.upload(chunk, stream);
+  mem_chunk_ptr.upload(ptr, stream);
+  mem_partial.alloc((size_t)std::max(nchunk, 1) * kMemQ);
+  mem_out.alloc((size_t)nmem * kMemQ);
+  PMX_HIP(hipStreamSynchronize(stream));   // the host tables above go out of scope
+}
+
+void OWModelDev::member_convergence(const double* p, double* out7, double* wells_out) {
+  PMX_CHECK(nmem > 0, "set the members before their convergence reductions");
+  PMX_CHECK(r.n >= S.n(), "assemble before the convergence reductions");
+  if (nchunk > 0) {
+    hipLaunchKernelGGL(k_ow_conv_chunks, dim3(nchunk), dim3(256), 0, stream, mem_chunk.p, r.p, p, pv.p, fluid,
+                       mem_partial.p);
+    PMX_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_ow_conv_members, dim3(nmem), dim3(64), 0, stream, mem_chunk_ptr.p, mem_partial.p,
+                     mem_out.p);
+  PMX_HIP(hipGetLastError());
+  PMX_HIP(hipMemcpyAsync(out7, mem_out.p, sizeof(double) * kMemQ * nmem, hipMemcpyDeviceToHost, stream));
+  if (nw > 0 && wells_out)
+    PMX_HIP(hipMemcpyAsync(wells_out, r.p + 2 * (size_t)nc, sizeof(double) * 3 * nw, hipMemcpyDeviceToHost, stream));
+  PMX_HIP(hipStreamSynchronize(stream));
 }
 
 void OWModelDev::mask_residual(const double* mask) {

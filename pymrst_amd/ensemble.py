@@ -47,12 +47,20 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
     def setup_members(self, cell_member: np.ndarray, well_member: np.ndarray, nmem: int):
         dev = self.device
         self.nmem = nmem
+        cell_member = np.asarray(cell_member, dtype=np.int64)
+        if cell_member.size and np.any(np.diff(cell_member) < 0):
+            raise ValueError("members' cells must be laid out one member after the other")
         self.cell_member = torch.as_tensor(cell_member, dtype=torch.long, device=dev)
         self.well_member = torch.as_tensor(well_member, dtype=torch.long, device=dev)
+        self.well_member_np = np.asarray(well_member, dtype=np.int64)
         self.pv_t = torch.as_tensor(self.pv, dtype=torch.float64, device=dev)
-        self.nc_member = torch.bincount(self.cell_member, minlength=nmem).to(torch.float64)
-        self.pv_member = torch.zeros(nmem, dtype=torch.float64, device=dev).index_add_(0, self.cell_member,
-                                                                                          self.pv_t)
+        counts = np.bincount(cell_member, minlength=nmem)
+        self.member_cell_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        self.nc_member = counts.astype(np.float64)
+        pv = np.asarray(self.pv, dtype=np.float64)
+        self.pv_member = np.array([pv[a:b].sum() for a, b in zip(self.member_cell_ptr[:-1],
+                                                                  self.member_cell_ptr[1:])])
+        _lib.check(self.lib.pmx_ow_set_members(self.h, nmem, _lib.ptr(self.member_cell_ptr)), "pmx_ow_set_members")
         qref = np.zeros(nmem)
         has_rate = np.zeros(nmem, bool)
         for k, w in enumerate(self.wells):
@@ -60,57 +68,54 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
                 m = well_member[k]
                 qref[m] = max(qref[m], abs(w.val))
                 has_rate[m] = True
-        qref = np.where(has_rate, np.maximum(qref, 1e-12), 1.0)
-        self.qref_member = torch.as_tensor(qref, device=dev)
-        self.well_is_bhp = torch.as_tensor([w.type == "bhp" for w in self.wells], device=dev)
+        self.qref_member = np.where(has_rate, np.maximum(qref, 1e-12), 1.0)
+        self.well_is_bhp = np.array([w.type == "bhp" for w in self.wells], dtype=bool)
         # unknown -> member, for freezing converged members
         self.unknown_member = torch.cat([self.cell_member.repeat_interleave(2),
                                          self.well_member.repeat_interleave(3)])
         self.member_its = np.zeros(nmem, dtype=np.int64)
         self._ok_m = None
+        self._ok_np = None
         self._frozen = None
         perf_member = np.repeat(np.asarray(well_member), [w.cells.size for w in self.wells]) \
             if self.nw else np.zeros(0, np.int64)
         self.perf_member = torch.as_tensor(perf_member, dtype=torch.long, device=dev)
 
-    def _seg(self, v, op, idx, n):
-        out = torch.zeros(n, dtype=v.dtype, device=v.device)
-        return out.scatter_reduce(0, idx, v, reduce=op, include_self=False)
-
     member_dt = None      # per-member step lengths (torch, nmem) while simulate_ensemble runs
 
     def converged(self, dt: float, st):
-        f = self.fluid
-        if self.member_dt is not None:
-            dt = self.member_dt
-        r = self.residual_device()
-        nc, M = self.nc, self.nmem
-        rw, ro = r[0:2 * nc:2], r[1:2 * nc:2]
-        p = st.p
-        inv_bw = torch.exp(-f.c[0] * (p - f.p_ref)) if f.c[0] else torch.ones_like(p)
-        inv_bo = torch.exp(-f.c[1] * (p - f.p_ref)) if f.c[1] else torch.ones_like(p)
-        cm = self.cell_member
-        Bw = self._seg(inv_bw, "sum", cm, M) / self.nc_member
-        Bo = self._seg(inv_bo, "sum", cm, M) / self.nc_member
-        cnv = torch.maximum(Bw * dt * self._seg((rw / self.pv_t).abs(), "amax", cm, M),
-                            Bo * dt * self._seg((ro / self.pv_t).abs(), "amax", cm, M))
-        mb = torch.maximum(Bw * dt * self._seg(rw, "sum", cm, M).abs() / self.pv_member,
-                           Bo * dt * self._seg(ro, "sum", cm, M).abs() / self.pv_member)
+        """The single model's criterion (_converged) for every member on its
+        own cells, wells and step length, from one device reduction per
+        member (pmx_ow_member_convergence); members already at the report
+        time count as converged."""
+        M = self.nmem
+        dt = self._member_dt_np if self.member_dt is not None else np.full(M, float(dt))
+        out = np.zeros((M, 7))
+        wr = np.zeros(3 * self.nw)
+        _lib.check(self.lib.pmx_ow_member_convergence(self.h, _lib.ptr(st.p), _lib.ptr(out),
+                                                      _lib.ptr(wr) if self.nw else None),
+                   "pmx_ow_member_convergence")
+        Bw = out[:, 4] / self.nc_member
+        Bo = out[:, 5] / self.nc_member
+        cnv = np.maximum(Bw * dt * out[:, 0], Bo * dt * out[:, 1])
+        mb = np.maximum(Bw * dt * np.abs(out[:, 2]) / self.pv_member, Bo * dt * np.abs(out[:, 3]) / self.pv_member)
         ok_m = (cnv <= self.tol_cnv) & (mb <= self.tol_mb)
         if self.nw:
-            wr = r[2 * nc:].view(-1, 3)
-            q = self.qref_member[self.well_member]
-            well_ok = (wr[:, 0].abs() <= self.tol_well * q) & (wr[:, 1].abs() <= self.tol_well * q)
-            scale = torch.where(self.well_is_bhp, torch.full_like(q, BARSA), q)
-            well_ok &= wr[:, 2].abs() <= self.tol_well * scale
-            bad = self._seg((~well_ok).to(torch.float64), "amax", self.well_member, M)
-            ok_m &= bad == 0
+            wr = wr.reshape(-1, 3)
+            q = self.qref_member[self.well_member_np]
+            well_ok = (np.abs(wr[:, 0]) <= self.tol_well * q) & (np.abs(wr[:, 1]) <= self.tol_well * q)
+            scale = np.where(self.well_is_bhp, BARSA, q)
+            well_ok &= np.abs(wr[:, 2]) <= self.tol_well * scale
+            bad = np.zeros(M, dtype=bool)
+            np.logical_or.at(bad, self.well_member_np, ~well_ok)
+            ok_m &= ~bad
         if self._frozen is not None:      # members already at the report time stay put
             ok_m |= self._frozen
         self.unconverged_members = int((~ok_m).sum())
         self.last_cnv_mb = (float(cnv.max()), float(mb.max()))
-        self._ok_m = ok_m
-        return bool(ok_m.all()), float(p.mean())
+        self._ok_np = ok_m
+        self._ok_m = torch.as_tensor(ok_m, device=self.device)
+        return bool(ok_m.all()), float(out[:, 6].sum() / self.nc)
 
     def solve_and_update(self, st, pmean: float) -> bool:
         """Members that have converged keep their state: their residual rows
@@ -119,7 +124,7 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
         if self._ok_m is not None:
             mask = (~self._ok_m)[self.unknown_member].to(torch.float64)
             _lib.check(self.lib.pmx_ow_mask_residual(self.h, _lib.ptr(mask)), "pmx_ow_mask_residual")
-            self.member_its += (~self._ok_m).cpu().numpy()
+            self.member_its += ~self._ok_np
         return super().solve_and_update(st, pmean)
 
     def reset(self):
@@ -132,6 +137,7 @@ class EnsembleModel(TwoPhaseOilWaterModelAD):
 
     # -- per-member time stepping ----------------------------------------------
     def _set_member_dt(self, dtm: np.ndarray):
+        self._member_dt_np = np.array(dtm, dtype=np.float64)
         self.member_dt = torch.as_tensor(dtm, dtype=torch.float64, device=self.device)
         self._dt_cell = self.member_dt[self.cell_member].contiguous()   # kept alive: the library reads it
         _lib.check(self.lib.pmx_ow_set_cell_dt(self.h, _lib.ptr(self._dt_cell)), "pmx_ow_set_cell_dt")
@@ -237,10 +243,10 @@ def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, vol, max_
             cdp = model._update_cdp(st0, restart)
             model._set_member_dt(dtm)
             restart[:] = False
-        model._frozen = torch.as_tensor(done, device=model.device)
+        model._frozen = done.copy()
         model.assemble_device(st0, st, float(D), cdp)
         _, pmean = model.converged(float(D), st)
-        ok = model._ok_m.cpu().numpy()
+        ok = model._ok_np
         accept = ok & ~done
         if accept.any():
             q = torch.zeros((M, 2), dtype=torch.float64, device=model.device)

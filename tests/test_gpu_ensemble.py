@@ -54,6 +54,56 @@ def test_member_convergence_is_per_member():
     assert not ok and model.unconverged_members == 1
 
 
+def test_member_convergence_reductions():
+    """pmx_ow_member_convergence against the same reductions of each
+    member's slice of the assembled residual (members of 126 to 3,000
+    cells, so one member spans several reduction chunks): maxima exact,
+    sums to rounding; and each member's whole-model reductions
+    (pmx_ow_convergence of the member built alone) agree."""
+    import ctypes
+    from pymrst_amd import _lib
+    from pymrst_amd.ensemble import build_ensemble
+    from pymrst_amd.ow import CPRSolver, DeviceOWState
+    from pymrst_amd.workloads import build_model, field_waterflood
+    cases = [field_waterflood(dims=d, seed=s, steps=1, step_days=10.0, inj_rate=2e-4)
+             for d, s in (((20, 15, 8), 1), ((7, 6, 3), 2), ((30, 20, 5), 3))]
+    model, s0, ranges = build_ensemble(cases, CPRSolver())
+    d0 = DeviceOWState.from_host(s0, model.device)
+    d = d0.copy()
+    g = torch.Generator(device=model.device).manual_seed(4)
+    d.p += 5e5 * torch.rand(d.p.shape, dtype=torch.float64, device=model.device, generator=g)
+    d.sw += 0.1 * torch.rand(d.sw.shape, dtype=torch.float64, device=model.device, generator=g)
+    dt = float(cases[0].dts[0])
+    model.assemble_device(d0, d, dt, model._cdp_dev)
+    out = np.zeros((3, 7))
+    wr = np.zeros(3 * model.nw)
+    _lib.check(model.lib.pmx_ow_member_convergence(model.h, _lib.ptr(d.p), _lib.ptr(out), _lib.ptr(wr)),
+               "pmx_ow_member_convergence")
+    r = model.residual_device().cpu().numpy()
+    p = d.p.cpu().numpy()
+    f = model.fluid
+    for m, (a, b) in enumerate(ranges):
+        rw, ro, pv = r[2 * a:2 * b:2], r[2 * a + 1:2 * b:2], model.pv[a:b]
+        assert out[m, 0] == np.abs(rw / pv).max()
+        assert out[m, 1] == np.abs(ro / pv).max()
+        np.testing.assert_allclose(out[m, 2:4], [rw.sum(), ro.sum()], rtol=1e-11, atol=1e-12 * np.abs(r).max())
+        ib = [np.exp(-c * (p[a:b] - f.p_ref)).sum() if c else float(b - a) for c in (f.c[0], f.c[1])]
+        np.testing.assert_allclose(out[m, 4:6], ib, rtol=1e-13)
+        np.testing.assert_allclose(out[m, 6], p[a:b].sum(), rtol=1e-13)
+    np.testing.assert_array_equal(wr, r[2 * model.nc:])
+    # each member alone, at the same states
+    for m, (c, (a, b)) in enumerate(zip(cases, ranges)):
+        mm, st0 = build_model(c, CPRSolver())
+        e0 = DeviceOWState.from_host(st0, mm.device)
+        e = e0.copy()
+        e.p.copy_(d.p[a:b])
+        e.sw.copy_(d.sw[a:b])
+        mm.assemble_device(e0, e, dt, mm._cdp_dev)
+        out7, _ = mm.convergence_values(e)
+        np.testing.assert_allclose(out[m, :2], out7[:2], rtol=1e-12)
+        np.testing.assert_allclose(out[m, 4:7], out7[4:7], rtol=1e-12)
+
+
 @pytest.mark.parametrize("compact", [0.0, 0.9])
 def test_per_member_time_stepping_matches_members_run_alone(compact):
     """simulate_ensemble: every member takes the ministeps, cuts and Newton
