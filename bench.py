@@ -263,7 +263,8 @@ def blackoil_leg(steps: int = 20, cpu: bool = True):
                                 spmv={"achieved_GBs": round(nbytes / avg_s / 1e9, 1) if prof["spmv_count"] else None,
                                       "avg_launch_us": round(avg_s * 1e6, 2), "launches": prof["spmv_count"],
                                       "bytes_per_launch": nbytes},
-                                saturated_cells_end=int((h.status == 3).sum()))
+                                saturated_cells_end=int((h.status == 3).sum()),
+                                launch_shapes=model.launch_shapes())
         print(f"[bench] black-oil FP{prec}: {newton} Newton its, {wall:.2f} s", file=sys.stderr, flush=True)
         del model
     out["workload"] = (f"black-oil {case.dims[0]}x{case.dims[1]}x{case.dims[2]} ({nc} cells), gas-cap injector "
@@ -369,7 +370,7 @@ def ensemble_leg(members: int = 256, loop_sample: int = 16, warmup: int = 1):
     tot_cells_its, tot_wall, ncl, loop_its = 0, 0.0, 0, []
     for k, c in enumerate(cases[:loop_sample]):
         print(f"[bench] ensemble: looped member {k} ({c.num_cells} cells)", file=sys.stderr, flush=True)
-        sv = CPRSolver(precision=64, maxiter=20, **OW_CPR)
+        sv = CPRSolver(precision=64, maxiter=20, **dict(OW_CPR, autotune=False))
         m, st0 = build_model(c, sv)
         m.max_iterations = 40
         if warmup:
@@ -477,11 +478,15 @@ def main():
                     help="CPR local stage of the oil-water legs (default: line Gauss-Seidel)")
     ap.add_argument("--line-segments", action="store_true",
                     help="solve lines of 32+ cells by segments with spikes (partitioned) instead of one walk")
+    ap.add_argument("--autotune", type=int, choices=(0, 1), default=1,
+                    help="1 (default): launch shapes chosen by timing at the first setup of each size class "
+                         "(AMG level layouts, folded launches, SpMV lanes); 0: fixed shapes")
     ap.add_argument("--no-incomp", action="store_true")
     ap.add_argument("--incomp-only", action="store_true")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
                     help="run only the SpMV and a calibration copy REPS times (for counter runs)")
     args = ap.parse_args()
+    OW_CPR["autotune"] = BO_CPR["autotune"] = bool(args.autotune)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
               "realisation or by y-slabs, which this build does not run across GPUs", file=sys.stderr)
@@ -528,6 +533,7 @@ def main():
         extra["newton_its_fp64"] = res[64]["its"].tolist()
         extra["speedup_mixed_vs_fp64"] = value / extra["fp64_value"]
     extra["newton_its"] = head["its"].tolist()
+    extra["launch_shapes"] = head["model"].launch_shapes()
     extra["gmres_its"] = head["lin_its"]
     prof = head["prof"]
     m = head["model"]

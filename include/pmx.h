@@ -47,6 +47,12 @@ typedef struct pmx_solver_options {
   int32_t coarse_block_rows; /* AMG levels up to this many rows run in one workgroup */
   int32_t line_segments;    /* 1: lines of 32+ cells solved by segments (spikes and a Schur system on
                                separator cells); 0: each line by one twisted walk (default) */
+  int32_t autotune;         /* 1: choose launch shapes by timing (each AMG level's sweep layout, lanes per
+                               row and folded launches; the SpMV's lanes per block row) at the first
+                               setup of each size class, kept in a process-wide table; 0: fixed shapes */
+  int32_t level_shape;      /* -1 (default): shapes as above; 0-6: every AMG level above the coarse cycle
+                               takes this shape (0 sweeps from the ELL copy where the level has one,
+                               1-3 CSR with 4, 8, 16 lanes per row, 4-6 the same with folded launches) */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {
@@ -176,6 +182,19 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4);
  * {launches timed, their total ms, their algorithmic bytes}.  No counterpart
  * in the reference (measurement only). */
 int pmx_ow_profile_line_gs(pmx_ow* h, double* out3);
+/* The launch shapes in use after a solve (pmx_solver_options.autotune):
+ * out[0] = the Jacobian SpMV's lanes per block row (0 before the first
+ * product), out[1] = number of AMG levels reported, then per level above
+ * the single-workgroup coarse cycle {rows, sweeps from the ELL copy (1) or
+ * CSR (0), lanes per CSR row, folded launches (1/0), tuned (1/0)};
+ * *count = ints written (at most cap).  No counterpart in the reference
+ * (launch configuration of this implementation). */
+int pmx_ow_launch_shapes(pmx_ow* h, int32_t* out, int32_t cap, int32_t* count);
+/* The process-wide autotuning table (size class -> chosen shape): number of
+ * entries / empty it, so that the next setups time their candidates again.
+ * No counterpart in the reference. */
+int pmx_autotune_entries(int32_t* count);
+int pmx_autotune_reset(void);
 
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);
@@ -327,6 +346,8 @@ int pmx_bo_export_blocks(pmx_bo* h, double* vals);
 /* Kernel timing as pmx_ow_profile / pmx_ow_profile_read. */
 int pmx_bo_profile(pmx_bo* h, int32_t enable);
 int pmx_bo_profile_read(pmx_bo* h, double* out4);
+/* pmx_ow_launch_shapes for the black-oil model's solver. */
+int pmx_bo_launch_shapes(pmx_bo* h, int32_t* out, int32_t cap, int32_t* count);
 
 /* ---- corner-point grids: MRST's processGRDECL topology
  *      (pymrst/gridprocessing/process_grdecl.py:352-580) on the host and

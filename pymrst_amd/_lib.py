@@ -7,6 +7,8 @@ is an error: there is no CPU fallback for the device path.
 import ctypes
 from pathlib import Path
 
+import numpy as np
+
 import torch  # noqa: F401  (loads the HIP runtime the library links against)
 
 _HERE = Path(__file__).resolve().parent
@@ -35,6 +37,8 @@ class SolverOptions(ctypes.Structure):
         ("use_graphs", ctypes.c_int32),
         ("coarse_block_rows", ctypes.c_int32),
         ("line_segments", ctypes.c_int32),
+        ("autotune", ctypes.c_int32),
+        ("level_shape", ctypes.c_int32),
     ]
 
 
@@ -80,6 +84,9 @@ SIGNATURES = {
     "pmx_ow_profile": (ctypes.c_int, [_P, _I]),
     "pmx_ow_profile_read": (ctypes.c_int, [_P, _P]),
     "pmx_ow_profile_line_gs": (ctypes.c_int, [_P, _P]),
+    "pmx_ow_launch_shapes": (ctypes.c_int, [_P, _P, _I, _P]),
+    "pmx_autotune_entries": (ctypes.c_int, [_P]),
+    "pmx_autotune_reset": (ctypes.c_int, []),
     "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
     "pmx_sys_create": (ctypes.c_int, [_I, _P, _I, _I, _P, _P, _I, _P, _P, ctypes.POINTER(_P)]),
     "pmx_sys_destroy": (ctypes.c_int, [_P]),
@@ -118,6 +125,7 @@ SIGNATURES = {
     "pmx_bo_export_blocks": (ctypes.c_int, [_P, _P]),
     "pmx_bo_profile": (ctypes.c_int, [_P, _I]),
     "pmx_bo_profile_read": (ctypes.c_int, [_P, _P]),
+    "pmx_bo_launch_shapes": (ctypes.c_int, [_P, _P, _I, _P]),
     "pmx_cp_process": (ctypes.c_int, [_I, _I, _I, _P, _P, _P, ctypes.POINTER(_P)]),
     "pmx_cp_destroy": (ctypes.c_int, [_P]),
     "pmx_cp_sizes": (ctypes.c_int, [_P, _P]),
@@ -185,3 +193,17 @@ def solver_defaults() -> SolverOptions:
     o = SolverOptions()
     load().pmx_solver_defaults(ctypes.byref(o))
     return o
+
+
+def read_launch_shapes(fn, h) -> dict:
+    """The launch shapes a model's solver uses (pmx_*_launch_shapes): the
+    SpMV's lanes per block row and, per AMG level above the coarse cycle,
+    its rows, sweep layout, lanes per CSR row and folding."""
+    out = np.zeros(512, np.int32)
+    n = ctypes.c_int32()
+    check(fn(h, ptr(out), out.size, ctypes.byref(n)), "launch_shapes")
+    out = out[:n.value]
+    levels = out[2:2 + 5 * int(out[1])].reshape(-1, 5)
+    return {"spmv_lanes": int(out[0]),
+            "levels": [{"rows": int(r), "layout": "ell" if e else "csr", "lanes": int(l), "folded": bool(f),
+                        "tuned": bool(t)} for r, e, l, f, t in levels]}

@@ -250,6 +250,9 @@ class ThreePhaseBlackOilModelAD:
     def profile(self, enable: bool):
         _lib.check(self.lib.pmx_bo_profile(self.h, int(enable)), "pmx_bo_profile")
 
+    def launch_shapes(self) -> dict:
+        return _lib.read_launch_shapes(self.lib.pmx_bo_launch_shapes, self.h)
+
     def profile_read(self) -> dict:
         out = np.zeros(4)
         _lib.check(self.lib.pmx_bo_profile_read(self.h, _lib.ptr(out)), "pmx_bo_profile_read")

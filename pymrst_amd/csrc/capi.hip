@@ -87,6 +87,8 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.use_graphs = o->use_graphs;
   r.coarse_block_rows = o->coarse_block_rows;
   r.line_segments = o->line_segments;
+  r.autotune = o->autotune;
+  r.level_shape = o->level_shape;
   return r;
 }
 
@@ -130,6 +132,8 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->use_graphs = d.use_graphs;
   o->coarse_block_rows = d.coarse_block_rows;
   o->line_segments = d.line_segments;
+  o->autotune = d.autotune;
+  o->level_shape = d.level_shape;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {
@@ -361,6 +365,24 @@ int pmx_ow_profile_read(pmx_ow* h, double* out4) {
     PMX_CHECK(h && out4, "null argument");
     bind(h);
     cpr_profile_read(h->m->cpr, out4);
+  })
+}
+
+int pmx_ow_launch_shapes(pmx_ow* h, int32_t* out, int32_t cap, int32_t* count) {
+  PMX_TRY({
+    PMX_CHECK(h && out && count && cap >= 2, "bad argument");
+    *count = cpr_launch_shapes(h->m->cpr, h->m->S, out, cap);
+  })
+}
+
+int pmx_autotune_reset(void) {
+  PMX_TRY({ tune_table_clear(); })
+}
+
+int pmx_autotune_entries(int32_t* count) {
+  PMX_TRY({
+    PMX_CHECK(count != nullptr, "null argument");
+    *count = tune_table_size();
   })
 }
 
@@ -848,6 +870,13 @@ int pmx_bo_profile(pmx_bo* h, int32_t enable) {
   PMX_TRY({
     PMX_CHECK(h, "null argument");
     cpr_profile(h->m->cpr, enable != 0);
+  })
+}
+
+int pmx_bo_launch_shapes(pmx_bo* h, int32_t* out, int32_t cap, int32_t* count) {
+  PMX_TRY({
+    PMX_CHECK(h && out && count && cap >= 2, "bad argument");
+    *count = cpr_launch_shapes(h->m->cpr, h->m->S, out, cap);
   })
 }
 

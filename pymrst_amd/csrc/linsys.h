@@ -36,6 +36,10 @@ struct BlockSystem {
   int npc = 0;
   DArr<double> Bp, Cp, Um, V, Dp;
   size_t n() const { return (size_t)nc * B + (size_t)nw * M; }
+  // lanes per block row of the SpMV: 0 until chosen (8 without autotuning;
+  // with it, the fastest of 4, 8, 16 timed at the first product)
+  mutable int spmv_lpr = 0;
+  mutable int autotune = 0;
   void set_pattern(const HostPattern& pat, hipStream_t s);
   void set_wells(int nw_, const std::vector<int>& perf_ptr, const std::vector<int>& perf_cells, hipStream_t s);
 };
@@ -81,6 +85,10 @@ struct LinearSolverOptions {
   int use_graphs = 1;        // replay the preconditioner application as a captured HIP graph
   int coarse_block_rows = 0;     // AMG levels up to this size run in one workgroup (the coarsest always does)
   int line_segments = 0;     // lines of 32+ cells solved by segments with spikes (measured slower: off)
+  int autotune = 0;          // time the launch-shape candidates of each AMG level and of the SpMV at the
+                             // first setup of each size class, keep the fastest (process-wide table)
+  int level_shape = -1;      // >= 0: every level above the coarse cycle takes this candidate shape
+                             // (0 ELL, 1-3 CSR with 4/8/16 lanes, 4-6 the same folded), for tests
 };
 
 struct SolveInfo {
@@ -132,6 +140,14 @@ void cpr_profile(CprSolver*, bool on);
 void cpr_profile_read(CprSolver*, double* out);
 // line-solve launches timed in eager applications: count, ms, algorithmic bytes
 void cpr_profile_line_read(CprSolver*, double* out3);
+// The launch shapes in use: out[0] = the SpMV's lanes per block row (0 before
+// the first product), out[1] = levels reported, then per level above the
+// coarse cycle {rows, ELL copy used, lanes per CSR row, folded, tuned};
+// returns the number of ints written (at most cap).
+int cpr_launch_shapes(CprSolver* C, const BlockSystem& S, int* out, int cap);
+// entries of the process-wide autotuning table / empty it
+int tune_table_size();
+void tune_table_clear();
 
 // Helpers used by several sources.
 int grid_for(size_t threads, int block = 256);

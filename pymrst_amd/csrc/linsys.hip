@@ -19,6 +19,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <type_traits>
 
 #include "linsys.h"
@@ -291,19 +294,122 @@ double EventTimer::total_ms(int* count) {
   return t;
 }
 
+// ---------------------------------------------------------------------------
+// Launch-shape autotuning: candidates timed with HIP events on the caller's
+// stream, the fastest kept per size class in a process-wide table (the
+// class: the kind of launch, its element size and block size, n and the
+// entries per row each in quarter-octave steps), so that a system of a size
+// already met reuses the choice without timing.  Never inside a capture.
+// ---------------------------------------------------------------------------
+struct TuneKey {
+  int kind, elem, nb, nz, extra;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(kind, elem, nb, nz, extra) < std::tie(o.kind, o.elem, o.nb, o.nz, o.extra);
+  }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, int> g_tune;
+
+static int octave4(double v) { return v > 0 ? (int)std::lround(4.0 * std::log2(v)) : -1; }
+
+static bool tune_lookup(const TuneKey& k, int* choice) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return false;
+  *choice = it->second;
+  return true;
+}
+
+static void tune_store(const TuneKey& k, int choice) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[k] = choice;
+}
+
+static bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  PMX_HIP(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Mean milliseconds of `reps` calls of f on s after one untimed call.
+template <class F>
+static float time_launches(hipStream_t s, int reps, F&& f) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PMX_HIP(hipEventCreate(&e0));
+  PMX_HIP(hipEventCreate(&e1));
+  f();
+  PMX_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) f();
+  PMX_HIP(hipEventRecord(e1, s));
+  PMX_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  PMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+int tune_table_size() {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return (int)g_tune.size();
+}
+
+void tune_table_clear() {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.clear();
+}
+
+template <int B, int LPR>
+static void spmv_launch(const BlockSystem& S, const double* x, double* y, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
+  if (e0)
+    hipExtLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, e0, e1, 0,
+                          S.nc, S.rowptr.p, S.col.p, S.val.p, x, y);
+  else
+    hipLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
+                       S.rowptr.p, S.col.p, S.val.p, x, y);
+  launch_check();
+}
+
+template <int B>
+static void spmv_any(int lpr, const BlockSystem& S, const double* x, double* y, hipStream_t s,
+                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (lpr == 4) spmv_launch<B, 4>(S, x, y, s, e0, e1);
+  else if (lpr == 16) spmv_launch<B, 16>(S, x, y, s, e0, e1);
+  else spmv_launch<B, 8>(S, x, y, s, e0, e1);
+}
+
+// The SpMV's lanes per block row: 8, or with autotuning the fastest of 4, 8
+// and 16 on this system (y is overwritten by the product that follows).
+template <int B>
+static int spmv_shape(const BlockSystem& S, const double* x, double* y, hipStream_t s) {
+  if (S.spmv_lpr) return S.spmv_lpr;
+  if (!S.autotune || stream_capturing(s)) return 8;
+  const TuneKey k{1, B, octave4(S.nc), octave4((double)S.nnzb / std::max(S.nc, 1)), 0};
+  int best = 8;
+  if (!tune_lookup(k, &best)) {
+    float tbest = 0.f;
+    for (int lpr : {4, 8, 16}) {
+      const float t = time_launches(s, 5, [&] { spmv_any<B>(lpr, S, x, y, s); });
+      if (lpr == 4 || t < tbest) {
+        tbest = t;
+        best = lpr;
+      }
+    }
+    tune_store(k, best);
+  }
+  S.spmv_lpr = best;
+  return best;
+}
+
 template <int B, int M, int X>
 static void matvec_t(const BlockSystem& S, const double* x, double* y, double* h, hipStream_t s, EventTimer* t) {
   const size_t ncB = (size_t)S.nc * B;
-  constexpr int LPR = 8;
   if (S.nc > 0) {
+    const int lpr = spmv_shape<B>(S, x, y, s);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (t && t->pair(&e0, &e1))
-      hipExtLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, e0, e1, 0,
-                            S.nc, S.rowptr.p, S.col.p, S.val.p, x, y);
-    else
-      hipLaunchKernelGGL((k_bcsr_spmv<B, LPR>), dim3(grid_for((size_t)S.nc * LPR)), dim3(256), 0, s, S.nc,
-                         S.rowptr.p, S.col.p, S.val.p, x, y);
-    launch_check();
+    if (t && t->pair(&e0, &e1)) spmv_any<B>(lpr, S, x, y, s, e0, e1);
+    else spmv_any<B>(lpr, S, x, y, s);
   }
   if (S.nw > 0) {
     hipLaunchKernelGGL((k_well_rows<B, M, X>), dim3(S.nw), dim3(64), 0, s, S.nw, S.well_ptr.p, S.perf_cell.p,
@@ -2099,6 +2205,91 @@ __global__ void k_prolong_add(int n, const int* __restrict__ agg, const T* __res
   if (i < n) x[i] += alpha * xc[agg[i]];
 }
 
+// Folded forms of a V-cycle level's launches (CSR, LPR lanes per row), for
+// levels where a launch costs more than the gathers it saves:
+// k_csr_jacobi2: the zero-guess Jacobi step and the first sweep,
+//   x0 = D^-1 b, out = x0 + D^-1 (b - A x0), with x0 formed where read;
+// k_csr_resid_restrict: the residual and its restriction,
+//   bc_I = sum over the aggregate's rows i of (b_i - (A x)_i);
+// k_csr_prolong_sweep: the coarse correction and the first post-sweep,
+//   y = x + alpha P xc formed where read, out = y + D^-1 (b - A y).
+// Each computes what the launches it replaces compute (the first and the
+// third in the same order at the same LPR; the second sums each
+// aggregate's residuals in another order).
+template <class T, int LPR>
+__global__ __launch_bounds__(256) void k_csr_jacobi2(int n, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ dinv, const T* __restrict__ b,
+                                                     T* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = t / LPR, lane = t % LPR;
+  T acc = 0;
+  if (row < n) {
+    const int e = rowptr[row + 1];
+    for (int k = rowptr[row] + lane; k < e; k += LPR) {
+      const int c = col[k];
+      const T x0 = dinv[c] * b[c];
+      acc = fma(val[k], x0, acc);
+    }
+  }
+  acc = wave_sum(acc, LPR);
+  if (row < n && lane == 0) {
+    const T x0 = dinv[row] * b[row];
+    const T res = b[row] - acc;
+    out[row] = fma(dinv[row], res, x0);
+  }
+}
+
+template <class T, int LPR>
+__global__ __launch_bounds__(256) void k_csr_resid_restrict(int nc, const int* __restrict__ aptr,
+                                                            const int* __restrict__ amem,
+                                                            const int* __restrict__ rowptr,
+                                                            const int* __restrict__ col, const T* __restrict__ val,
+                                                            const T* __restrict__ b, const T* __restrict__ x,
+                                                            T* __restrict__ bc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int I = t / LPR, lane = t % LPR;
+  T acc = 0;
+  if (I < nc) {
+    for (int m = aptr[I] + lane; m < aptr[I + 1]; m += LPR) {
+      const int i = amem[m];
+      T ax = 0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) ax = fma(val[k], x[col[k]], ax);
+      acc += b[i] - ax;
+    }
+  }
+  acc = wave_sum(acc, LPR);
+  if (I < nc && lane == 0) bc[I] = acc;
+}
+
+template <class T, int LPR>
+__global__ __launch_bounds__(256) void k_csr_prolong_sweep(int n, const int* __restrict__ rowptr,
+                                                           const int* __restrict__ col, const T* __restrict__ val,
+                                                           const T* __restrict__ dinv, const T* __restrict__ b,
+                                                           const T* __restrict__ x, const int* __restrict__ agg,
+                                                           const T* __restrict__ xc, T alpha,
+                                                           T* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = t / LPR, lane = t % LPR;
+  T acc = 0;
+  if (row < n) {
+    const int e = rowptr[row + 1];
+    for (int k = rowptr[row] + lane; k < e; k += LPR) {
+      const int c = col[k];
+      T y = x[c];
+      y += alpha * xc[agg[c]];
+      acc = fma(val[k], y, acc);
+    }
+  }
+  acc = wave_sum(acc, LPR);
+  if (row < n && lane == 0) {
+    T y = x[row];
+    y += alpha * xc[agg[row]];
+    const T res = b[row] - acc;
+    out[row] = fma(dinv[row], res, y);
+  }
+}
+
 __global__ void k_dense_fill(int n, const int* __restrict__ rowptr, const int* __restrict__ col,
                              const double* __restrict__ val64, const float* __restrict__ val32,
                              double* __restrict__ dense) {
@@ -2261,6 +2452,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarse_cycle(int nlev, const
 // 23.8 Mcell-Newton-its/s over the run); levels of 140k-200k rows do not
 // (their sweeps stay latency-bound), hence the threshold.
 constexpr int kSellRows = 262144;
+// with autotuning, levels from this size keep an ELL copy as a candidate
+constexpr int kSellTuneRows = 65536;
 struct AmgLevel {
   int n = 0, nnz = 0, ncoarse = 0;
   DArr<int> rowptr, col, diag, agg, agg_ptr, agg_mem, gal_ptr, gal_src;
@@ -2271,7 +2464,24 @@ struct AmgLevel {
   DArr<int> sl_ptr, sl_len, sl_col, sl_map;
   DArr<double> sv64;
   DArr<float> sv32;
+  // launch shapes of this level's V-cycle launches: lanes per CSR row, the
+  // ELL copy (when the level has one) or CSR, and folded launches (CSR);
+  // fixed defaults, or chosen by timing at the first setup when autotuned
+  int lpr = 8;
+  bool use_sell = true;
+  bool fold = false;
+  bool tuned = false;
 };
+
+static bool sweeps_from_sell(const AmgLevel& a) { return a.nslice > 0 && a.use_sell; }
+
+// Candidate shapes of a level: 0 the ELL copy (CSR with 8 lanes where the
+// level has none), 1-3 CSR with 4, 8, 16 lanes per row, 4-6 the same folded.
+static void set_level_shape(AmgLevel& a, int cand) {
+  a.use_sell = cand == 0;
+  a.lpr = cand == 0 ? 8 : (4 << ((cand - 1) % 3));
+  a.fold = cand >= 4;
+}
 
 struct SegFGMRES;
 int cpr_solve_segmented(CprSolver* C, const BlockSystem& S, const double* b, double* x,
@@ -2368,6 +2578,19 @@ void cpr_profile_line_read(CprSolver* C, double* out) {
   out[2] = C->lgs_bytes;
 }
 
+int cpr_launch_shapes(CprSolver* C, const BlockSystem& S, int* out, int cap) {
+  std::vector<int> v{S.spmv_lpr, 0};
+  const int nL = (int)C->L.size();
+  for (int l = 0; l + 1 < nL && l < C->coarse_l0; ++l) {
+    const AmgLevel& a = *C->L[l];
+    v.insert(v.end(), {a.n, sweeps_from_sell(a) ? 1 : 0, a.lpr, a.fold ? 1 : 0, a.tuned ? 1 : 0});
+    ++v[1];
+  }
+  const int n = std::min(cap, (int)v.size());
+  std::copy(v.begin(), v.begin() + n, out);
+  return n;
+}
+
 void cpr_profile_read(CprSolver* C, double* out) {
   int n1 = 0, n2 = 0;
   out[1] = C->spmv_t.total_ms(&n1);
@@ -2416,7 +2639,7 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
     lv->col.upload(h.pat.col, s);
     lv->diag.upload(h.pat.diag, s);
     lv->ncoarse = h.ncoarse;
-    if (l + 1 < H.size() && lv->n >= kSellRows) {
+    if (l + 1 < H.size() && lv->n >= (o.autotune ? kSellTuneRows : kSellRows)) {
       const int n = lv->n, ns = (n + 63) / 64;
       const std::vector<int>& rp = h.pat.rowptr;
       std::vector<int> ptr(ns + 1), len(ns), map(lv->nnz);
@@ -2619,6 +2842,9 @@ struct Lv<float> {
   static DArr<float>& pacc(CprSolver& c) { return c.pacc32; }
 };
 
+template <class T>
+static void tune_levels(CprSolver* C, const LinearSolverOptions& o, hipStream_t s);
+
 template <int B, int M, int X, class T>
 static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, const LinearSolverOptions& o,
                     hipStream_t s) {
@@ -2779,6 +3005,20 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
   }
   C->coarse_desc.upload(reinterpret_cast<const char*>(desc.data()), sizeof(CoarseLevel<T>) * desc.size(), s);
   C->coarse_l0 = l0;
+  S.autotune = o.autotune;
+  if (o.level_shape >= 0) {
+    PMX_CHECK(o.level_shape <= 6, "level_shape is -1 or a candidate 0-6");
+    for (auto& lp : C->L) set_level_shape(*lp, o.level_shape);
+  } else if (o.autotune) {
+    if (!stream_capturing(s)) tune_levels<T>(C, o, s);
+  } else {
+    for (auto& lp : C->L) {
+      lp->lpr = 8;
+      lp->use_sell = true;
+      lp->fold = false;
+      lp->tuned = false;
+    }
+  }
   PMX_HIP(hipStreamSynchronize(s));
   C->prec_setup = sizeof(T) == 8 ? 64 : 32;
   C->graph_ok = false;
@@ -2806,19 +3046,103 @@ void cpr_setup(CprSolver* C, const BlockSystem& S, const double* w_dev, const do
     PMX_DISPATCH(S, (setup_t<B_, M_, X_, double>(C, S, w_dev, o, s)));
 }
 
+// Calls f with the lanes-per-row constant for lpr (4, 8 or 16).
+template <class F>
+static void with_lpr(int lpr, F&& f) {
+  if (lpr == 4) f(std::integral_constant<int, 4>{});
+  else if (lpr == 16) f(std::integral_constant<int, 16>{});
+  else f(std::integral_constant<int, 8>{});
+}
+
 // One Jacobi sweep (mode 0: out = x + D^-1 (b - A x)) or residual (mode 1:
-// out = b - A x) on level a, from its ELL copy when it has one.
+// out = b - A x) on level a, from its ELL copy when it has one and uses it.
 template <class T>
 static void level_sweep(AmgLevel& a, const T* b, const T* x, T* out, int mode, hipStream_t s) {
   using V = Lv<T>;
-  constexpr int LPR = 8;
-  if (a.nslice > 0)
+  if (sweeps_from_sell(a))
     hipLaunchKernelGGL((k_sell_sweep<T>), dim3(grid_for(a.n)), dim3(256), 0, s, a.n, a.sl_ptr.p, a.sl_len.p,
                        a.sl_col.p, V::sv(a).p, V::d(a).p, b, x, out, mode);
   else
-    hipLaunchKernelGGL((k_csr_sweep<T, LPR>), dim3(grid_for((size_t)a.n * LPR)), dim3(256), 0, s, a.n,
-                       a.rowptr.p, a.col.p, V::v(a).p, V::d(a).p, b, x, out, mode);
+    with_lpr(a.lpr, [&](auto c) {
+      constexpr int LPR = decltype(c)::value;
+      hipLaunchKernelGGL((k_csr_sweep<T, LPR>), dim3(grid_for((size_t)a.n * LPR)), dim3(256), 0, s, a.n,
+                         a.rowptr.p, a.col.p, V::v(a).p, V::d(a).p, b, x, out, mode);
+    });
   launch_check();
+}
+
+// The launches of level l on the way down (pre-smoothing from a zero guess,
+// residual, restriction into level l+1's right-hand side) and on the way up
+// (coarse correction, post-smoothing), in their folded forms when the level
+// uses them.
+template <class T>
+static void level_down(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStream_t s) {
+  using V = Lv<T>;
+  AmgLevel& a = *C->L[l];
+  AmgLevel& c = *C->L[l + 1];
+  const int n = a.n;
+  const bool fold = a.fold && !sweeps_from_sell(a);
+  int it = 1;
+  if (fold && o.pre_smooth >= 2) {
+    with_lpr(a.lpr, [&](auto k) {
+      constexpr int LPR = decltype(k)::value;
+      hipLaunchKernelGGL((k_csr_jacobi2<T, LPR>), dim3(grid_for((size_t)n * LPR)), dim3(256), 0, s, n,
+                         a.rowptr.p, a.col.p, V::v(a).p, V::d(a).p, V::b(a).p, V::y(a).p);
+    });
+    launch_check();
+    std::swap(V::x(a).p, V::y(a).p);
+    it = 2;
+  } else {
+    hipLaunchKernelGGL((k_jacobi0<T>), dim3(grid_for(n)), dim3(256), 0, s, n, V::d(a).p, V::b(a).p, V::x(a).p);
+    launch_check();
+  }
+  for (; it < o.pre_smooth; ++it) {
+    level_sweep<T>(a, V::b(a).p, V::x(a).p, V::y(a).p, 0, s);
+    std::swap(V::x(a).p, V::y(a).p);
+  }
+  if (fold) {
+    with_lpr(a.lpr, [&](auto k) {
+      constexpr int LPR = decltype(k)::value;
+      hipLaunchKernelGGL((k_csr_resid_restrict<T, LPR>), dim3(grid_for((size_t)c.n * LPR)), dim3(256), 0, s, c.n,
+                         a.agg_ptr.p, a.agg_mem.p, a.rowptr.p, a.col.p, V::v(a).p, V::b(a).p, V::x(a).p,
+                         V::b(c).p);
+    });
+    launch_check();
+  } else {
+    level_sweep<T>(a, V::b(a).p, V::x(a).p, V::r(a).p, 1, s);
+    hipLaunchKernelGGL((k_restrict_agg<T>), dim3(grid_for(c.n)), dim3(256), 0, s, c.n, a.agg_ptr.p, a.agg_mem.p,
+                       V::r(a).p, V::b(c).p);
+    launch_check();
+  }
+}
+
+template <class T>
+static void level_up(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStream_t s) {
+  using V = Lv<T>;
+  AmgLevel& a = *C->L[l];
+  AmgLevel& c = *C->L[l + 1];
+  const int n = a.n;
+  const bool fold = a.fold && !sweeps_from_sell(a);
+  int it = 0;
+  if (fold && o.post_smooth >= 1) {
+    with_lpr(a.lpr, [&](auto k) {
+      constexpr int LPR = decltype(k)::value;
+      hipLaunchKernelGGL((k_csr_prolong_sweep<T, LPR>), dim3(grid_for((size_t)n * LPR)), dim3(256), 0, s, n,
+                         a.rowptr.p, a.col.p, V::v(a).p, V::d(a).p, V::b(a).p, V::x(a).p, a.agg.p, V::x(c).p,
+                         (T)o.coarse_alpha, V::y(a).p);
+    });
+    launch_check();
+    std::swap(V::x(a).p, V::y(a).p);
+    it = 1;
+  } else {
+    hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n)), dim3(256), 0, s, n, a.agg.p, V::x(c).p,
+                       (T)o.coarse_alpha, V::x(a).p);
+    launch_check();
+  }
+  for (; it < o.post_smooth; ++it) {
+    level_sweep<T>(a, V::b(a).p, V::x(a).p, V::y(a).p, 0, s);
+    std::swap(V::x(a).p, V::y(a).p);
+  }
 }
 
 // One V-cycle on level l: x_l = V(b_l) (zero initial guess).
@@ -2826,7 +3150,6 @@ template <class T>
 static void vcycle(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStream_t s) {
   using V = Lv<T>;
   AmgLevel& a = *C->L[l];
-  constexpr int LPR = 8;
   const int n = a.n;
   if ((int)l == C->coarse_l0) {
     hipLaunchKernelGGL((k_coarse_cycle<T>), dim3(1), dim3(kCoarseThreads), 0, s, (int)C->L.size() - (int)l,
@@ -2841,27 +3164,47 @@ static void vcycle(CprSolver* C, size_t l, const LinearSolverOptions& o, hipStre
     launch_check();
     return;
   }
-  AmgLevel& c = *C->L[l + 1];
-  const int gs = grid_for((size_t)n * LPR);
-  // pre-smoothing (x = 0 initially)
-  hipLaunchKernelGGL((k_jacobi0<T>), dim3(grid_for(n)), dim3(256), 0, s, n, V::d(a).p, V::b(a).p, V::x(a).p);
-  launch_check();
-  for (int it = 1; it < o.pre_smooth; ++it) {
-    level_sweep<T>(a, V::b(a).p, V::x(a).p, V::y(a).p, 0, s);
-    std::swap(V::x(a).p, V::y(a).p);
-  }
-  // residual and restriction
-  level_sweep<T>(a, V::b(a).p, V::x(a).p, V::r(a).p, 1, s);
-  hipLaunchKernelGGL((k_restrict_agg<T>), dim3(grid_for(c.n)), dim3(256), 0, s, c.n, a.agg_ptr.p, a.agg_mem.p,
-                     V::r(a).p, V::b(c).p);
-  launch_check();
-  for (int cyc = 0; cyc < (l + 2 == C->L.size() ? 1 : 1); ++cyc) vcycle<T>(C, l + 1, o, s);
-  hipLaunchKernelGGL((k_prolong_add<T>), dim3(grid_for(n)), dim3(256), 0, s, n, a.agg.p, V::x(c).p,
-                     (T)o.coarse_alpha, V::x(a).p);
-  launch_check();
-  for (int it = 0; it < o.post_smooth; ++it) {
-    level_sweep<T>(a, V::b(a).p, V::x(a).p, V::y(a).p, 0, s);
-    std::swap(V::x(a).p, V::y(a).p);
+  level_down<T>(C, l, o, s);
+  vcycle<T>(C, l + 1, o, s);
+  level_up<T>(C, l, o, s);
+}
+
+// Autotuning of the levels above the coarse cycle: each level's down and up
+// launches (with level l+1's vectors as they stand) timed for every shape
+// (the ELL copy when the level has one; CSR with 4, 8 or 16 lanes per row,
+// unfolded and folded) and the fastest kept.  The vectors are set to zero
+// first (the timings do not depend on the values) and are overwritten by
+// the first application.  Choices are kept per size class.
+template <class T>
+static void tune_levels(CprSolver* C, const LinearSolverOptions& o, hipStream_t s) {
+  using V = Lv<T>;
+  const int nL = (int)C->L.size();
+  for (int l = 0; l + 1 < nL && l < C->coarse_l0; ++l) {
+    AmgLevel& a = *C->L[l];
+    if (a.tuned) continue;
+    AmgLevel& c = *C->L[l + 1];
+    const TuneKey k{2, (int)sizeof(T), octave4(a.n), octave4((double)a.nnz / std::max(a.n, 1)),
+                    (a.nslice > 0 ? 1 : 0) + 2 * o.pre_smooth + 16 * o.post_smooth};
+    int choice = -1;
+    if (!tune_lookup(k, &choice)) {
+      for (DArr<T>* v : {&V::b(a), &V::x(a), &V::y(a), &V::r(a), &V::b(c), &V::x(c)})
+        PMX_HIP(hipMemsetAsync(v->p, 0, sizeof(T) * v->n, s));
+      float tbest = 0.f;
+      for (int cand = (a.nslice > 0 ? 0 : 1); cand < 7; ++cand) {
+        set_level_shape(a, cand);
+        const float t = time_launches(s, 3, [&] {
+          level_down<T>(C, l, o, s);
+          level_up<T>(C, l, o, s);
+        });
+        if (choice < 0 || t < tbest) {
+          tbest = t;
+          choice = cand;
+        }
+      }
+      tune_store(k, choice);
+    }
+    set_level_shape(a, choice);
+    a.tuned = true;
   }
 }
 

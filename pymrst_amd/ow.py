@@ -116,6 +116,10 @@ class CPRSolver:
     use_graphs: bool = True
     coarse_block_rows: int = 0
     line_segments: bool = False
+    # launch shapes (AMG level layouts and folding, SpMV lanes per row) chosen
+    # by timing at the first setup of each size class
+    autotune: bool = False
+    level_shape: int = -1      # 0-6: one launch shape for every AMG level (testing)
     # a pressure-stage solver with solve_linear_system(A_p, r) -> x, called on
     # the equilibrated A_p = W J P in every preconditioner application
     # instead of the device AMG (linear_solvers.py:124-131, 253-256)
@@ -168,6 +172,8 @@ class CPRSolver:
         o.use_graphs = int(self.use_graphs)
         o.coarse_block_rows = self.coarse_block_rows
         o.line_segments = 1 if self.line_segments else 0
+        o.autotune = 1 if self.autotune else 0
+        o.level_shape = self.level_shape
         return o
 
 
@@ -295,6 +301,9 @@ class TwoPhaseOilWaterModelAD:
 
     def profile(self, enable: bool):
         _lib.check(self.lib.pmx_ow_profile(self.h, int(enable)), "pmx_ow_profile")
+
+    def launch_shapes(self) -> dict:
+        return _lib.read_launch_shapes(self.lib.pmx_ow_launch_shapes, self.h)
 
     def profile_read(self) -> dict:
         out = np.zeros(4)
