@@ -203,7 +203,8 @@ def read_launch_shapes(fn, h) -> dict:
     n = ctypes.c_int32()
     check(fn(h, ptr(out), out.size, ctypes.byref(n)), "launch_shapes")
     out = out[:n.value]
-    levels = out[2:2 + 5 * int(out[1])].reshape(-1, 5)
-    return {"spmv_lanes": int(out[0]),
+    nl = int(out[1])
+    levels = out[2:2 + 5 * nl].reshape(-1, 5)
+    return {"spmv_lanes": int(out[0]), "projection_elems": int(out[2 + 5 * nl]) if out.size > 2 + 5 * nl else 0,
             "levels": [{"rows": int(r), "layout": "ell" if e else "csr", "lanes": int(l), "folded": bool(f),
                         "tuned": bool(t)} for r, e, l, f, t in levels]}

@@ -2531,6 +2531,8 @@ class CprSolver {
   DArr<float> lf_inv32, lf_lo32, lf_up32, r2_32, xpl32, yc32, r1l32, aoff32, lf_wl32, lf_wr32;
   // coarse levels handled by one workgroup: first such level, descriptions
   int coarse_l0 = -1;
+  int dot2_elems = 0;                 // elements per thread of k_multidot2 (0: not chosen yet)
+  size_t dot2_n = 0;                  // the system length it was chosen for
   DArr<char> coarse_desc;
   // captured preconditioner application
   PrecIO* io = nullptr;
@@ -2586,6 +2588,7 @@ int cpr_launch_shapes(CprSolver* C, const BlockSystem& S, int* out, int cap) {
     v.insert(v.end(), {a.n, sweeps_from_sell(a) ? 1 : 0, a.lpr, a.fold ? 1 : 0, a.tuned ? 1 : 0});
     ++v[1];
   }
+  v.push_back(C->dot2_elems);
   const int n = std::min(cap, (int)v.size());
   std::copy(v.begin(), v.begin() + n, out);
   return n;
@@ -3496,6 +3499,49 @@ static bool hopeless(double beta, double& best, int& since_best) {
   return ++since_best >= kHopelessCycles;
 }
 
+static int dot2_blocks(size_t n, int e) {
+  const size_t tile = (size_t)kRedThreads * e;
+  return (int)((n + tile - 1) / tile);
+}
+
+static void multidot2_any(int e, int nblk, hipStream_t s, size_t n, int nv, const double* V, size_t ld,
+                          const double* a, const double* w, double* partial) {
+  if (e == 2)
+    hipLaunchKernelGGL(k_multidot2<2>, dim3(nblk), dim3(kRedThreads), 0, s, n, nv, V, ld, a, w, partial, nullptr,
+                       nullptr);
+  else if (e == 4)
+    hipLaunchKernelGGL(k_multidot2<4>, dim3(nblk), dim3(kRedThreads), 0, s, n, nv, V, ld, a, w, partial, nullptr,
+                       nullptr);
+  else
+    hipLaunchKernelGGL(k_multidot2<8>, dim3(nblk), dim3(kRedThreads), 0, s, n, nv, V, ld, a, w, partial, nullptr,
+                       nullptr);
+  launch_check();
+}
+
+// Elements per thread of the two-sided projections: kDot2Elems, or with
+// autotuning the fastest of 2, 4 and 8 on this system's length, timed over
+// half the basis (columns 2.. of the workspace, values irrelevant; the
+// partial sums are overwritten by the solve).
+static int dot2_shape(CprSolver* C, size_t n, int m, size_t ld, hipStream_t s) {
+  if (stream_capturing(s) || m < 4) return kDot2Elems;
+  const int nv = m / 2;
+  const TuneKey k{3, 8, octave4((double)n), octave4((double)nv), 0};
+  int best = kDot2Elems;
+  if (tune_lookup(k, &best)) return best;
+  float tbest = 0.f;
+  for (int e : {2, 4, 8}) {
+    const float t = time_launches(s, 5, [&] {
+      multidot2_any(e, dot2_blocks(n, e), s, n, nv, C->V.p + 2 * ld, ld, C->V.p, C->V.p + ld, C->partial.p);
+    });
+    if (e == 2 || t < tbest) {
+      tbest = t;
+      best = e;
+    }
+  }
+  tune_store(k, best);
+  return best;
+}
+
 int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, const LinearSolverOptions& o,
               SolveInfo& info, hipStream_t s) {
   if (C->seg) return cpr_solve_segmented(C, S, b, x, o, info, s);
@@ -3508,7 +3554,8 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
     C->Z.alloc(ld * m);
     C->w.alloc(ld);
     C->r.alloc(ld);
-    C->partial.alloc((size_t)red_blocks(n) * (2 * m + 4) + 2 * (2 * m + 4));
+    // room for the smallest tile of k_multidot2 (kRedThreads * 2 elements)
+    C->partial.alloc((size_t)dot2_blocks(n, 2) * (2 * m + 4) + 2 * (2 * m + 4));
     C->hdev.alloc(2 * m + 4);
     C->ydev.alloc(2 * m + 4);
     C->hmat.alloc((size_t)std::max(S.nw, 1) * kMaxMix);
@@ -3519,7 +3566,12 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   double* hscr = C->hmat.p;
   const int nb = grid_for(n);
   const int rb = red_blocks(n);
-  const int rb2 = (int)((n + (size_t)kRedThreads * kDot2Elems - 1) / ((size_t)kRedThreads * kDot2Elems));
+  if (C->dot2_elems == 0 || C->dot2_n != n || (!o.autotune && C->dot2_elems != kDot2Elems)) {
+    C->dot2_elems = o.autotune ? dot2_shape(C, n, m, ld, s) : kDot2Elems;
+    C->dot2_n = n;
+  }
+  const int e2dot = C->dot2_elems;
+  const int rb2 = dot2_blocks(n, e2dot);
   // ||b||
   const double bnorm = std::sqrt(std::max(0.0, device_dot(b, b, n, C->partial.p, s)));
   PMX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
@@ -3620,9 +3672,7 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
         launch_check();
       } else {
         const int nd = 2 * j + 3;
-        hipLaunchKernelGGL(k_multidot2<kDot2Elems>, dim3(rb2), dim3(kRedThreads), 0, s, n, j, C->V.p, ld, vj, w,
-                           C->partial.p, nullptr, nullptr);
-        launch_check();
+        multidot2_any(e2dot, rb2, s, n, j, C->V.p, ld, vj, w, C->partial.p);
         reduce_to_host(nd, hh, rb2);
         const double* sv = hh.data();
         const double* tv = hh.data() + j;
