@@ -2615,6 +2615,7 @@ void cpr_set_lines(CprSolver* C, int nlines, const int* ptr, const int* cells) {
 
 void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<double>& weights,
                          const LinearSolverOptions& o, hipStream_t s) {
+  const auto t_build = std::chrono::steady_clock::now();
   // host copy of the BCSR pattern
   HostPattern fine;
   fine.n = S.nc;
@@ -2776,6 +2777,8 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
   C->graph_ok = false;
   C->prec_setup = 0;
   if (o.verbose) {
+    fprintf(stderr, "[pmx] hierarchy and lines built in %.1f ms\n",
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count());
     fprintf(stderr, "[pmx] AMG hierarchy:");
     for (auto& lv : C->L) fprintf(stderr, " %d(%d)", lv->n, lv->nnz);
     fprintf(stderr, "\n[pmx] lines: %d in %d colours, longest %d\n", C->nlines, C->ncolor, C->longest);
@@ -3013,7 +3016,13 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     PMX_CHECK(o.level_shape <= 6, "level_shape is -1 or a candidate 0-6");
     for (auto& lp : C->L) set_level_shape(*lp, o.level_shape);
   } else if (o.autotune) {
-    if (!stream_capturing(s)) tune_levels<T>(C, o, s);
+    if (!stream_capturing(s)) {
+      const auto t_tune = std::chrono::steady_clock::now();
+      tune_levels<T>(C, o, s);
+      if (o.verbose)
+        fprintf(stderr, "[pmx] level shapes tuned in %.1f ms\n",
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_tune).count());
+    }
   } else {
     for (auto& lp : C->L) {
       lp->lpr = 8;

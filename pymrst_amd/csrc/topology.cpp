@@ -39,10 +39,13 @@ void tpfa_pattern(int n, int nconn, const int* N, HostPattern& pat, std::vector<
   entry_conn.resize(nnz);
   pat.diag.resize(n);
   for (int i = 0; i < n; ++i) {
-    auto b0 = ent.begin() + pat.rowptr[i], b1 = ent.begin() + pat.rowptr[i + 1];
-    std::stable_sort(b0, b1, [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
-      return x.first < y.first;
-    });
+    // stable insertion sort by column (rows are short; no temporary buffer)
+    for (int k = pat.rowptr[i] + 1; k < pat.rowptr[i + 1]; ++k) {
+      const std::pair<int, int> e = ent[k];
+      int q = k;
+      for (; q > pat.rowptr[i] && ent[q - 1].first > e.first; --q) ent[q] = ent[q - 1];
+      ent[q] = e;
+    }
     for (int k = pat.rowptr[i]; k < pat.rowptr[i + 1]; ++k) {
       // one connection per block entry: the assembly reads one T and one
       // gravity drop per entry.  Two connections between the same cells
@@ -152,10 +155,13 @@ static int aggregate_pairwise(const HostPattern& P, const std::vector<double>& w
 }
 
 // Coarse pattern, Galerkin gather map (for piecewise-constant prolongation)
-// and coarse strength weights.
+// and coarse strength weights.  Flat arrays throughout: each coarse row's
+// columns are gathered once (a mark per column), sorted in place, and every
+// fine entry finds its coarse entry through the row's slot table.
 static void coarsen(const HostPattern& P, const std::vector<double>& w, const std::vector<int>& agg, int nc,
                     HostLevel& L, HostPattern& CP, std::vector<double>& cw) {
   const int n = P.n;
+  const int nnz = P.rowptr[n];
   L.agg = agg;
   L.ncoarse = nc;
   L.agg_ptr.assign(nc + 1, 0);
@@ -168,56 +174,48 @@ static void coarsen(const HostPattern& P, const std::vector<double>& w, const st
   }
   CP.n = nc;
   CP.rowptr.assign(nc + 1, 0);
-  std::vector<int> cols;
-  std::vector<int> mark(nc, -1);
-  std::vector<std::vector<int>> rows(nc);
+  CP.diag.resize(nc);
+  CP.col.clear();
+  CP.col.reserve((size_t)nc + (size_t)nnz / 2);
+  std::vector<int> mark(nc, -1), slot(nc, 0), dest(nnz);
   for (int I = 0; I < nc; ++I) {
-    auto& r = rows[I];
-    r.push_back(I);
+    const size_t r0 = CP.col.size();
+    CP.col.push_back(I);
     mark[I] = I;
     for (int m = L.agg_ptr[I]; m < L.agg_ptr[I + 1]; ++m) {
-      int i = L.agg_mem[m];
+      const int i = L.agg_mem[m];
       for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
-        int J = agg[P.col[k]];
-        if (mark[J] != I) { mark[J] = I; r.push_back(J); }
+        const int J = agg[P.col[k]];
+        if (mark[J] != I) {
+          mark[J] = I;
+          CP.col.push_back(J);
+        }
       }
     }
-    std::sort(r.begin(), r.end());
-    CP.rowptr[I + 1] = CP.rowptr[I] + (int)r.size();
-  }
-  int cnnz = CP.rowptr[nc];
-  CP.col.resize(cnnz);
-  CP.diag.resize(nc);
-  for (int I = 0; I < nc; ++I) {
-    std::copy(rows[I].begin(), rows[I].end(), CP.col.begin() + CP.rowptr[I]);
-    for (int K = CP.rowptr[I]; K < CP.rowptr[I + 1]; ++K)
-      if (CP.col[K] == I) CP.diag[I] = K;
-    std::vector<int>().swap(rows[I]);
-  }
-  // gather map
-  std::vector<int> dest(P.rowptr[n]);
-  L.gal_ptr.assign(cnnz + 1, 0);
-  for (int I = 0; I < nc; ++I)
+    std::sort(CP.col.begin() + r0, CP.col.end());
+    for (size_t q = r0; q < CP.col.size(); ++q) slot[CP.col[q]] = (int)q;
+    CP.diag[I] = slot[I];
     for (int m = L.agg_ptr[I]; m < L.agg_ptr[I + 1]; ++m) {
-      int i = L.agg_mem[m];
-      for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
-        int J = agg[P.col[k]];
-        auto b0 = CP.col.begin() + CP.rowptr[I], b1 = CP.col.begin() + CP.rowptr[I + 1];
-        int K = (int)(std::lower_bound(b0, b1, J) - CP.col.begin());
-        dest[k] = K;
-        L.gal_ptr[K + 1]++;
-      }
+      const int i = L.agg_mem[m];
+      for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) dest[k] = slot[agg[P.col[k]]];
     }
+    CP.rowptr[I + 1] = (int)CP.col.size();
+  }
+  const int cnnz = CP.rowptr[nc];
+  // gather map: coarse entry K <- the fine entries k with dest[k] == K, in
+  // coarse-row, member, entry order
+  L.gal_ptr.assign(cnnz + 1, 0);
+  for (int k = 0; k < nnz; ++k) L.gal_ptr[dest[k] + 1]++;
   for (int K = 0; K < cnnz; ++K) L.gal_ptr[K + 1] += L.gal_ptr[K];
-  L.gal_src.resize(P.rowptr[n]);
+  L.gal_src.resize(nnz);
   cw.assign(cnnz, 0.0);
   {
     std::vector<int> f(L.gal_ptr.begin(), L.gal_ptr.end() - 1);
     for (int I = 0; I < nc; ++I)
       for (int m = L.agg_ptr[I]; m < L.agg_ptr[I + 1]; ++m) {
-        int i = L.agg_mem[m];
+        const int i = L.agg_mem[m];
         for (int k = P.rowptr[i]; k < P.rowptr[i + 1]; ++k) {
-          int K = dest[k];
+          const int K = dest[k];
           L.gal_src[f[K]++] = k;
           if (CP.col[K] != I && P.col[k] != i) cw[K] += w[k];
         }
