@@ -481,6 +481,9 @@ def main():
     ap.add_argument("--autotune", type=int, choices=(0, 1), default=1,
                     help="1 (default): launch shapes chosen by timing at the first setup of each size class "
                          "(AMG level layouts, folded launches, SpMV lanes); 0: fixed shapes")
+    ap.add_argument("--tune-file", default=None, metavar="PATH",
+                    help="launch-shape table: read at the start when the file exists (those shapes are used "
+                         "without timing them), written at the end")
     ap.add_argument("--no-incomp", action="store_true")
     ap.add_argument("--incomp-only", action="store_true")
     ap.add_argument("--traffic-probe", type=int, default=0, metavar="REPS",
@@ -495,6 +498,13 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         return 2
+    if args.tune_file:
+        import atexit
+        from pymrst_amd import _lib
+        if os.path.exists(args.tune_file):
+            n = _lib.autotune_load(args.tune_file)
+            print(f"[bench] launch shapes: {n} entries read from {args.tune_file}", file=sys.stderr, flush=True)
+        atexit.register(_lib.autotune_save, args.tune_file)
     from pymrst_amd.workloads import field_waterflood
     dims = tuple(int(v) for v in args.dims.split(","))
     case = field_waterflood(dims=dims)

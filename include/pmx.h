@@ -196,6 +196,13 @@ int pmx_ow_launch_shapes(pmx_ow* h, int32_t* out, int32_t cap, int32_t* count);
  * No counterpart in the reference. */
 int pmx_autotune_entries(int32_t* count);
 int pmx_autotune_reset(void);
+/* The table as a text file (one entry per line: launch kind, element size,
+ * rows class, entries-per-row class, extra, choice): written, or read back
+ * so that another process uses the same shapes without timing them (a run
+ * under a profiler, whose per-dispatch overhead would bias the timings).
+ * *count (may be NULL) = entries written / read. */
+int pmx_autotune_save(const char* path, int32_t* count);
+int pmx_autotune_load(const char* path, int32_t* count);
 
 /* One preconditioner application z = M^{-1} r after a solve's setup (device). */
 int pmx_ow_precondition(pmx_ow* h, const pmx_solver_options* o, const double* r, double* z);

@@ -87,6 +87,8 @@ SIGNATURES = {
     "pmx_ow_launch_shapes": (ctypes.c_int, [_P, _P, _I, _P]),
     "pmx_autotune_entries": (ctypes.c_int, [_P]),
     "pmx_autotune_reset": (ctypes.c_int, []),
+    "pmx_autotune_save": (ctypes.c_int, [ctypes.c_char_p, _P]),
+    "pmx_autotune_load": (ctypes.c_int, [ctypes.c_char_p, _P]),
     "pmx_ow_precondition": (ctypes.c_int, [_P, ctypes.POINTER(SolverOptions), _P, _P]),
     "pmx_sys_create": (ctypes.c_int, [_I, _P, _I, _I, _P, _P, _I, _P, _P, ctypes.POINTER(_P)]),
     "pmx_sys_destroy": (ctypes.c_int, [_P]),
@@ -208,3 +210,20 @@ def read_launch_shapes(fn, h) -> dict:
     return {"spmv_lanes": int(out[0]), "projection_elems": int(out[2 + 5 * nl]) if out.size > 2 + 5 * nl else 0,
             "levels": [{"rows": int(r), "layout": "ell" if e else "csr", "lanes": int(l), "folded": bool(f),
                         "tuned": bool(t)} for r, e, l, f, t in levels]}
+
+
+def autotune_save(path) -> int:
+    """Write the process-wide launch-shape table (pmx_autotune_save); returns
+    the number of entries."""
+    n = ctypes.c_int32()
+    check(load().pmx_autotune_save(str(path).encode(), ctypes.byref(n)), "pmx_autotune_save")
+    return n.value
+
+
+def autotune_load(path) -> int:
+    """Read a table written by autotune_save into the process-wide table
+    (pmx_autotune_load): later setups of those size classes use its shapes
+    without timing; returns the number of entries read."""
+    n = ctypes.c_int32()
+    check(load().pmx_autotune_load(str(path).encode(), ctypes.byref(n)), "pmx_autotune_load")
+    return n.value

@@ -379,6 +379,22 @@ int pmx_autotune_reset(void) {
   PMX_TRY({ tune_table_clear(); })
 }
 
+int pmx_autotune_save(const char* path, int32_t* count) {
+  PMX_TRY({
+    PMX_CHECK(path != nullptr, "null path");
+    const int n = tune_table_save(path);
+    if (count) *count = n;
+  })
+}
+
+int pmx_autotune_load(const char* path, int32_t* count) {
+  PMX_TRY({
+    PMX_CHECK(path != nullptr, "null path");
+    const int n = tune_table_load(path);
+    if (count) *count = n;
+  })
+}
+
 int pmx_autotune_entries(int32_t* count) {
   PMX_TRY({
     PMX_CHECK(count != nullptr, "null argument");

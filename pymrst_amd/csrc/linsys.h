@@ -149,6 +149,8 @@ int cpr_launch_shapes(CprSolver* C, const BlockSystem& S, int* out, int cap);
 // entries of the process-wide autotuning table / empty it
 int tune_table_size();
 void tune_table_clear();
+int tune_table_save(const char* path);
+int tune_table_load(const char* path);
 
 // Helpers used by several sources.
 int grid_for(size_t threads, int block = 256);

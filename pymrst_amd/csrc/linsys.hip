@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -349,6 +350,19 @@ static float time_launches(hipStream_t s, int reps, F&& f) {
   return ms / reps;
 }
 
+// A candidate replaces the fixed default shape only when it is faster by
+// more than this fraction (choices stay put under timing noise).
+constexpr float kTuneMargin = 0.03f;
+
+// The fastest of `times` (candidate i at times[i]) unless the default
+// candidate `def` is within kTuneMargin of it.
+static int tune_pick(const std::vector<float>& times, int def) {
+  int best = def;
+  for (int i = 0; i < (int)times.size(); ++i)
+    if (times[i] < times[best]) best = i;
+  return times[best] < (1.0f - kTuneMargin) * times[def] ? best : def;
+}
+
 int tune_table_size() {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   return (int)g_tune.size();
@@ -357,6 +371,37 @@ int tune_table_size() {
 void tune_table_clear() {
   std::lock_guard<std::mutex> lock(g_tune_mu);
   g_tune.clear();
+}
+
+// The table as text, one entry per line: kind, element size, rows class,
+// entries-per-row class, extra, choice.
+int tune_table_save(const char* path) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  FILE* f = std::fopen(path, "w");
+  PMX_CHECK(f != nullptr, std::string("cannot write the tuning table to ") + path);
+  for (const auto& [k, c] : g_tune) std::fprintf(f, "%d %d %d %d %d %d\n", k.kind, k.elem, k.nb, k.nz, k.extra, c);
+  const bool ok = std::fclose(f) == 0;
+  PMX_CHECK(ok, std::string("cannot write the tuning table to ") + path);
+  return (int)g_tune.size();
+}
+
+// Entries read from a table written by tune_table_save replace those of the
+// same class; returns the number read.
+int tune_table_load(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  PMX_CHECK(f != nullptr, std::string("cannot read the tuning table ") + path);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  int n = 0;
+  TuneKey k{};
+  int c = 0, got = 0;
+  while ((got = std::fscanf(f, "%d %d %d %d %d %d", &k.kind, &k.elem, &k.nb, &k.nz, &k.extra, &c)) == 6) {
+    g_tune[k] = c;
+    ++n;
+  }
+  const bool clean = got == EOF;
+  std::fclose(f);
+  PMX_CHECK(clean, std::string("malformed tuning table ") + path);
+  return n;
 }
 
 template <int B, int LPR>
@@ -388,14 +433,10 @@ static int spmv_shape(const BlockSystem& S, const double* x, double* y, hipStrea
   const TuneKey k{1, B, octave4(S.nc), octave4((double)S.nnzb / std::max(S.nc, 1)), 0};
   int best = 8;
   if (!tune_lookup(k, &best)) {
-    float tbest = 0.f;
-    for (int lpr : {4, 8, 16}) {
-      const float t = time_launches(s, 5, [&] { spmv_any<B>(lpr, S, x, y, s); });
-      if (lpr == 4 || t < tbest) {
-        tbest = t;
-        best = lpr;
-      }
-    }
+    const int cand[3] = {4, 8, 16};
+    std::vector<float> t;
+    for (int lpr : cand) t.push_back(time_launches(s, 5, [&] { spmv_any<B>(lpr, S, x, y, s); }));
+    best = cand[tune_pick(t, 1)];
     tune_store(k, best);
   }
   S.spmv_lpr = best;
@@ -3201,18 +3242,17 @@ static void tune_levels(CprSolver* C, const LinearSolverOptions& o, hipStream_t 
     if (!tune_lookup(k, &choice)) {
       for (DArr<T>* v : {&V::b(a), &V::x(a), &V::y(a), &V::r(a), &V::b(c), &V::x(c)})
         PMX_HIP(hipMemsetAsync(v->p, 0, sizeof(T) * v->n, s));
-      float tbest = 0.f;
+      // candidate 0 (ELL) only where the level has the copy; the default is
+      // ELL there and CSR with 8 lanes elsewhere
+      std::vector<float> t(7, std::numeric_limits<float>::max());
       for (int cand = (a.nslice > 0 ? 0 : 1); cand < 7; ++cand) {
         set_level_shape(a, cand);
-        const float t = time_launches(s, 3, [&] {
+        t[cand] = time_launches(s, 5, [&] {
           level_down<T>(C, l, o, s);
           level_up<T>(C, l, o, s);
         });
-        if (choice < 0 || t < tbest) {
-          tbest = t;
-          choice = cand;
-        }
       }
+      choice = tune_pick(t, a.nslice > 0 ? 0 : 2);
       tune_store(k, choice);
     }
     set_level_shape(a, choice);
@@ -3537,16 +3577,13 @@ static int dot2_shape(CprSolver* C, size_t n, int m, size_t ld, hipStream_t s) {
   const TuneKey k{3, 8, octave4((double)n), octave4((double)nv), 0};
   int best = kDot2Elems;
   if (tune_lookup(k, &best)) return best;
-  float tbest = 0.f;
-  for (int e : {2, 4, 8}) {
-    const float t = time_launches(s, 5, [&] {
+  const int cand[3] = {2, 4, 8};
+  std::vector<float> t;
+  for (int e : cand)
+    t.push_back(time_launches(s, 5, [&] {
       multidot2_any(e, dot2_blocks(n, e), s, n, nv, C->V.p + 2 * ld, ld, C->V.p, C->V.p + ld, C->partial.p);
-    });
-    if (e == 2 || t < tbest) {
-      tbest = t;
-      best = e;
-    }
-  }
+    }));
+  best = cand[tune_pick(t, 2)];
   tune_store(k, best);
   return best;
 }
