@@ -2149,13 +2149,20 @@ __global__ void k_ilu_upper_level(int nr, const int* __restrict__ rows, const in
 // ---------------------------------------------------------------------------
 // Aggregation AMG kernels (scalar CSR, value type T)
 // ---------------------------------------------------------------------------
+// Coarse values as sums of fine values over the gather map; with rr, each
+// fine value is also stored at its gather-map position (the fine level's
+// entries grouped by coarse row, for the folded residual-and-restriction).
 template <class T>
 __global__ void k_galerkin(int cnnz, const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                           const T* __restrict__ fine, T* __restrict__ coarse) {
+                           const T* __restrict__ fine, T* __restrict__ coarse, T* __restrict__ rr) {
   const int K = blockIdx.x * blockDim.x + threadIdx.x;
   if (K >= cnnz) return;
   double s = 0.0;
-  for (int m = gptr[K]; m < gptr[K + 1]; ++m) s += (double)fine[gsrc[m]];
+  for (int m = gptr[K]; m < gptr[K + 1]; ++m) {
+    const T v = fine[gsrc[m]];
+    s += (double)v;
+    if (rr) rr[m] = v;
+  }
   coarse[K] = (T)s;
 }
 
@@ -2251,7 +2258,8 @@ __global__ void k_prolong_add(int n, const int* __restrict__ agg, const T* __res
 // k_csr_jacobi2: the zero-guess Jacobi step and the first sweep,
 //   x0 = D^-1 b, out = x0 + D^-1 (b - A x0), with x0 formed where read;
 // k_csr_resid_restrict: the residual and its restriction,
-//   bc_I = sum over the aggregate's rows i of (b_i - (A x)_i);
+//   bc_I = sum over the aggregate's rows i of (b_i - (A x)_i), over the
+//   aggregate's entries through the Galerkin gather map;
 // k_csr_prolong_sweep: the coarse correction and the first post-sweep,
 //   y = x + alpha P xc formed where read, out = y + D^-1 (b - A y).
 // Each computes what the launches it replaces compute (the first and the
@@ -2281,23 +2289,26 @@ __global__ __launch_bounds__(256) void k_csr_jacobi2(int n, const int* __restric
   }
 }
 
+// The fine entries of coarse row I are one contiguous range of the Galerkin
+// gather map, [gptr[crow[I]], gptr[crow[I+1]]) (every entry of every row of
+// the aggregate), stored in that order with their columns (rr_col, rr_val),
+// so the lanes of a coarse row read them coalesced and split them evenly,
+// whatever the aggregate's size.
 template <class T, int LPR>
 __global__ __launch_bounds__(256) void k_csr_resid_restrict(int nc, const int* __restrict__ aptr,
                                                             const int* __restrict__ amem,
-                                                            const int* __restrict__ rowptr,
-                                                            const int* __restrict__ col, const T* __restrict__ val,
-                                                            const T* __restrict__ b, const T* __restrict__ x,
-                                                            T* __restrict__ bc) {
+                                                            const int* __restrict__ crow,
+                                                            const int* __restrict__ gptr,
+                                                            const int* __restrict__ rr_col,
+                                                            const T* __restrict__ rr_val, const T* __restrict__ b,
+                                                            const T* __restrict__ x, T* __restrict__ bc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int I = t / LPR, lane = t % LPR;
   T acc = 0;
   if (I < nc) {
-    for (int m = aptr[I] + lane; m < aptr[I + 1]; m += LPR) {
-      const int i = amem[m];
-      T ax = 0;
-      for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) ax = fma(val[k], x[col[k]], ax);
-      acc += b[i] - ax;
-    }
+    for (int m = aptr[I] + lane; m < aptr[I + 1]; m += LPR) acc += b[amem[m]];
+    const int e1 = gptr[crow[I + 1]];
+    for (int e = gptr[crow[I]] + lane; e < e1; e += LPR) acc = fma(-rr_val[e], x[rr_col[e]], acc);
   }
   acc = wave_sum(acc, LPR);
   if (I < nc && lane == 0) bc[I] = acc;
@@ -2512,6 +2523,11 @@ struct AmgLevel {
   bool use_sell = true;
   bool fold = false;
   bool tuned = false;
+  // for folding: the fine entries grouped by coarse row (gather-map order):
+  // their columns (static) and values (per setup)
+  DArr<int> rr_col;
+  DArr<double> rr64;
+  DArr<float> rr32;
 };
 
 static bool sweeps_from_sell(const AmgLevel& a) { return a.nslice > 0 && a.use_sell; }
@@ -2722,6 +2738,12 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
       lv->agg_mem.upload(h.agg_mem, s);
       lv->gal_ptr.upload(h.gal_ptr, s);
       lv->gal_src.upload(h.gal_src, s);
+      // columns of the fine entries in gather-map order (folded launches)
+      if (o.autotune || o.level_shape >= 4) {
+        std::vector<int> rc(h.gal_src.size());
+        for (size_t e = 0; e < rc.size(); ++e) rc[e] = h.pat.col[h.gal_src[e]];
+        lv->rr_col.upload(rc, s);
+      }
     }
     C->L.push_back(std::move(lv));
   }
@@ -2834,6 +2856,7 @@ template <>
 struct Lv<double> {
   static DArr<double>& v(AmgLevel& l) { return l.v64; }
   static DArr<double>& sv(AmgLevel& l) { return l.sv64; }
+  static DArr<double>& rr(AmgLevel& l) { return l.rr64; }
   static DArr<double>& d(AmgLevel& l) { return l.d64; }
   static DArr<double>& x(AmgLevel& l) { return l.x64; }
   static DArr<double>& y(AmgLevel& l) { return l.y64; }
@@ -2863,6 +2886,7 @@ template <>
 struct Lv<float> {
   static DArr<float>& v(AmgLevel& l) { return l.v32; }
   static DArr<float>& sv(AmgLevel& l) { return l.sv32; }
+  static DArr<float>& rr(AmgLevel& l) { return l.rr32; }
   static DArr<float>& d(AmgLevel& l) { return l.d32; }
   static DArr<float>& x(AmgLevel& l) { return l.x32; }
   static DArr<float>& y(AmgLevel& l) { return l.y32; }
@@ -3006,8 +3030,15 @@ static void setup_t(CprSolver* C, const BlockSystem& S, const double* wdev, cons
     }
     if (l + 1 < C->L.size()) {
       AmgLevel& c = *C->L[l + 1];
+      // the fine values in gather-map order for the folded residual-and-
+      // restriction, on levels that can fold (CSR sweeps)
+      T* rr = nullptr;
+      if (a.rr_col.n > 0) {
+        V::rr(a).alloc(a.rr_col.n);
+        rr = V::rr(a).p;
+      }
       hipLaunchKernelGGL((k_galerkin<T>), dim3(grid_for(c.nnz)), dim3(256), 0, s, c.nnz, a.gal_ptr.p, a.gal_src.p,
-                         V::v(a).p, V::v(c).p);
+                         V::v(a).p, V::v(c).p, rr);
       launch_check();
       hipLaunchKernelGGL((k_diag_inv<T>), dim3(grid_for(a.n)), dim3(256), 0, s, a.n, a.diag.p, V::v(a).p,
                          o.jacobi_omega, V::d(a).p);
@@ -3153,12 +3184,12 @@ static void level_down(CprSolver* C, size_t l, const LinearSolverOptions& o, hip
     level_sweep<T>(a, V::b(a).p, V::x(a).p, V::y(a).p, 0, s);
     std::swap(V::x(a).p, V::y(a).p);
   }
-  if (fold) {
+  if (fold && a.rr_col.n > 0 && V::rr(a).n == a.rr_col.n) {
     with_lpr(a.lpr, [&](auto k) {
       constexpr int LPR = decltype(k)::value;
       hipLaunchKernelGGL((k_csr_resid_restrict<T, LPR>), dim3(grid_for((size_t)c.n * LPR)), dim3(256), 0, s, c.n,
-                         a.agg_ptr.p, a.agg_mem.p, a.rowptr.p, a.col.p, V::v(a).p, V::b(a).p, V::x(a).p,
-                         V::b(c).p);
+                         a.agg_ptr.p, a.agg_mem.p, c.rowptr.p, a.gal_ptr.p, a.rr_col.p, V::rr(a).p, V::b(a).p,
+                         V::x(a).p, V::b(c).p);
     });
     launch_check();
   } else {
