@@ -187,7 +187,8 @@ int pmx_ow_profile_line_gs(pmx_ow* h, double* out3);
  * product), out[1] = number of AMG levels reported, then per level above
  * the single-workgroup coarse cycle {rows, sweeps from the ELL copy (1) or
  * CSR (0), lanes per CSR row, folded launches (1/0), tuned (1/0)}, then the
- * Gram-Schmidt projections' elements per thread (0 before the first solve);
+ * Gram-Schmidt projections' shape (2, 4, 8: elements per thread with
+ * shuffle sums; 104, 108: 4 or 8 with LDS sums; 0 before the first solve);
  * *count = ints written (at most cap).  No counterpart in the reference
  * (launch configuration of this implementation). */
 int pmx_ow_launch_shapes(pmx_ow* h, int32_t* out, int32_t cap, int32_t* count);

@@ -143,7 +143,8 @@ void cpr_profile_line_read(CprSolver*, double* out3);
 // The launch shapes in use: out[0] = the SpMV's lanes per block row (0 before
 // the first product), out[1] = levels reported, then per level above the
 // coarse cycle {rows, ELL copy used, lanes per CSR row, folded, tuned}, then
-// the elements per thread of the projections (0 before the first solve);
+// the projections' shape (elements per thread; +100 for the LDS-sum form;
+// 0 before the first solve);
 // returns the number of ints written (at most cap).
 int cpr_launch_shapes(CprSolver* C, const BlockSystem& S, int* out, int cap);
 // entries of the process-wide autotuning table / empty it

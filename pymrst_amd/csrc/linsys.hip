@@ -705,6 +705,88 @@ __global__ __launch_bounds__(kRedThreads) void k_multidot2(size_t n, int nv, con
   }
 }
 
+// The same projections with the cross-lane sums done through LDS: each
+// pass over kLdsGroup basis vectors leaves 2 kLdsGroup partial dots per
+// thread, stored transposed in LDS; then 16 threads reduce each dot (16
+// values each, conflict-free reads, then a 16-lane shuffle tree), one
+// barrier pair per pass instead of a 64-lane shuffle tree per dot.
+constexpr int kLdsGroup = 8;
+template <int E>
+__global__ __launch_bounds__(kRedThreads) void k_multidot2_lds(size_t n, int nv, const double* __restrict__ V,
+                                                               size_t ld, const double* __restrict__ a,
+                                                               const double* __restrict__ w,
+                                                               double* __restrict__ partial) {
+  static_assert(2 * kLdsGroup * 16 == kRedThreads, "16 threads per dot of a pass");
+  __shared__ double sh[2 * kLdsGroup][kRedThreads];
+  __shared__ double s3[kRedThreads / 64][3];
+  const int nd = 2 * nv + 3;
+  constexpr size_t tile = (size_t)kRedThreads * E;
+  const size_t t0 = (size_t)blockIdx.x * tile;
+  const size_t len = n - t0 < tile ? n - t0 : tile;
+  const size_t base = t0 + threadIdx.x;
+  const size_t end = t0 + len;
+  const int tid = threadIdx.x;
+  double av[E], wv[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const size_t i = base + (size_t)e * kRedThreads;
+    av[e] = i < end ? a[i] : 0.0;
+    wv[e] = i < end ? w[i] : 0.0;
+  }
+  const int q = tid >> 4, r = tid & 15;   // this thread's dot in the reduction, and its slice
+  for (int d0 = 0; d0 < nv; d0 += kLdsGroup) {
+#pragma unroll
+    for (int g = 0; g < kLdsGroup; ++g) {
+      const int d = d0 + g;
+      double sa = 0.0, sw = 0.0;
+      if (d < nv) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const size_t i = base + (size_t)e * kRedThreads;
+          if (i < end) {
+            const double v = V[(size_t)d * ld + i];
+            sa = fma(v, av[e], sa);
+            sw = fma(v, wv[e], sw);
+          }
+        }
+      }
+      sh[g][tid] = sa;
+      sh[kLdsGroup + g][tid] = sw;
+    }
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += sh[q][k * 16 + r];
+    t = wave_sum(t, 16);
+    const int side = q / kLdsGroup, g = q % kLdsGroup;
+    if (r == 0 && d0 + g < nv) partial[(size_t)blockIdx.x * nd + side * nv + d0 + g] = t;
+    __syncthreads();
+  }
+  double aa = 0.0, aw = 0.0, ww = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    aa = fma(av[e], av[e], aa);
+    aw = fma(av[e], wv[e], aw);
+    ww = fma(wv[e], wv[e], ww);
+  }
+  aa = wave_sum(aa, 64);
+  aw = wave_sum(aw, 64);
+  ww = wave_sum(ww, 64);
+  const int lane = tid & 63, wid = tid >> 6;
+  if (lane == 0) {
+    s3[wid][0] = aa;
+    s3[wid][1] = aw;
+    s3[wid][2] = ww;
+  }
+  __syncthreads();
+  if (tid < 3) {
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < kRedThreads / 64; ++v) t += s3[v][tid];
+    partial[(size_t)blockIdx.x * nd + 2 * nv + tid] = t;
+  }
+}
+
 // Delayed reorthogonalisation step, one read of the basis:
 //   a <- (a - sum_d s_d V_d) * inv_nu                       (coef[0..nv))
 //   w <- (w - sum_d c_d V_d - a_coef * a_old) * scale       (coef[nv..2nv))
@@ -3579,14 +3661,20 @@ static bool hopeless(double beta, double& best, int& since_best) {
   return ++since_best >= kHopelessCycles;
 }
 
+// shape codes of the projections: 2, 4, 8 elements per thread with shuffle
+// sums; 104, 108: 4 or 8 elements per thread with LDS sums
 static int dot2_blocks(size_t n, int e) {
-  const size_t tile = (size_t)kRedThreads * e;
+  const size_t tile = (size_t)kRedThreads * (e % 100);
   return (int)((n + tile - 1) / tile);
 }
 
 static void multidot2_any(int e, int nblk, hipStream_t s, size_t n, int nv, const double* V, size_t ld,
                           const double* a, const double* w, double* partial) {
-  if (e == 2)
+  if (e == 104)
+    hipLaunchKernelGGL(k_multidot2_lds<4>, dim3(nblk), dim3(kRedThreads), 0, s, n, nv, V, ld, a, w, partial);
+  else if (e == 108)
+    hipLaunchKernelGGL(k_multidot2_lds<8>, dim3(nblk), dim3(kRedThreads), 0, s, n, nv, V, ld, a, w, partial);
+  else if (e == 2)
     hipLaunchKernelGGL(k_multidot2<2>, dim3(nblk), dim3(kRedThreads), 0, s, n, nv, V, ld, a, w, partial, nullptr,
                        nullptr);
   else if (e == 4)
@@ -3598,22 +3686,52 @@ static void multidot2_any(int e, int nblk, hipStream_t s, size_t n, int nv, cons
   launch_check();
 }
 
+__global__ void k_fill_probe(size_t n, double* __restrict__ x) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (double)((i * 2654435761u) % 1000u) / 1000.0 - 0.5;
+}
+
 // Elements per thread of the two-sided projections: kDot2Elems, or with
-// autotuning the fastest of 2, 4 and 8 on this system's length, timed over
-// half the basis (columns 2.. of the workspace, values irrelevant; the
-// partial sums are overwritten by the solve).
+// autotuning the fastest of 2, 4 and 8 with shuffle sums and 4 and 8 with
+// LDS sums on this system's length, timed over half the basis on a fixed
+// pattern written into the workspace (the solve overwrites it).  Each
+// candidate's dots are checked against the default's first; one that
+// differs by more than 1e-12 relative is not taken.
 static int dot2_shape(CprSolver* C, size_t n, int m, size_t ld, hipStream_t s) {
   if (stream_capturing(s) || m < 4) return kDot2Elems;
-  const int nv = m / 2;
+  const int nv = m / 2, nd = 2 * nv + 3;
   const TuneKey k{3, 8, octave4((double)n), octave4((double)nv), 0};
   int best = kDot2Elems;
   if (tune_lookup(k, &best)) return best;
-  const int cand[3] = {2, 4, 8};
+  hipLaunchKernelGGL(k_fill_probe, dim3(grid_for(ld * (nv + 2))), dim3(256), 0, s, ld * (nv + 2), C->V.p);
+  launch_check();
+  auto dots = [&](int e) {
+    multidot2_any(e, dot2_blocks(n, e), s, n, nv, C->V.p + 2 * ld, ld, C->V.p, C->V.p + ld, C->partial.p);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nd), dim3(256), 0, s, dot2_blocks(n, e), nd, C->partial.p,
+                       C->hdev.p);
+    launch_check();
+    std::vector<double> h(nd);
+    PMX_HIP(hipMemcpyAsync(h.data(), C->hdev.p, sizeof(double) * nd, hipMemcpyDeviceToHost, s));
+    PMX_HIP(hipStreamSynchronize(s));
+    return h;
+  };
+  const std::vector<double> ref = dots(kDot2Elems);
+  double scale = 0.0;
+  for (double v : ref) scale = std::max(scale, std::fabs(v));
+  const int cand[5] = {2, 4, 8, 104, 108};
   std::vector<float> t;
-  for (int e : cand)
+  for (int e : cand) {
+    const std::vector<double> h = dots(e);
+    double err = 0.0;
+    for (int i = 0; i < nd; ++i) err = std::max(err, std::fabs(h[i] - ref[i]));
+    if (!(err <= 1e-12 * scale)) {
+      t.push_back(std::numeric_limits<float>::max());
+      continue;
+    }
     t.push_back(time_launches(s, 5, [&] {
       multidot2_any(e, dot2_blocks(n, e), s, n, nv, C->V.p + 2 * ld, ld, C->V.p, C->V.p + ld, C->partial.p);
     }));
+  }
   best = cand[tune_pick(t, 2)];
   tune_store(k, best);
   return best;
