@@ -2306,6 +2306,28 @@ __global__ __launch_bounds__(256) void k_sell_sweep(int n, const int* __restrict
   out[row] = mode == 0 ? x[row] + dinv[row] * res : res;
 }
 
+// The ELL sweep with 16-bit column offsets (column = row + offset): 6 bytes
+// per stored FP32 entry instead of 8.
+template <class T>
+__global__ __launch_bounds__(256) void k_sell_sweep16(int n, const int* __restrict__ sl_ptr,
+                                                      const int* __restrict__ sl_len,
+                                                      const int16_t* __restrict__ sl_off, const T* __restrict__ sv,
+                                                      const T* __restrict__ dinv, const T* __restrict__ b,
+                                                      const T* __restrict__ x, T* __restrict__ out, int mode) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int sl = row >> 6;
+  const int base = sl_ptr[sl] + (row & 63), len = sl_len[sl];
+  T acc = 0;
+#pragma unroll 8
+  for (int k = 0; k < len; ++k) {
+    const int q = base + 64 * k;
+    acc = fma(sv[q], x[row + (int)sl_off[q]], acc);
+  }
+  const T res = b[row] - acc;
+  out[row] = mode == 0 ? x[row] + dinv[row] * res : res;
+}
+
 template <class T>
 __global__ void k_to_sell(int nnz, const int* __restrict__ map, const T* __restrict__ v, T* __restrict__ sv) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2595,6 +2617,8 @@ struct AmgLevel {
   DArr<float> v32, d32, x32, y32, b32, r32;
   int nslice = 0;
   size_t sell_slots = 0;
+  bool sell16 = false;                // sl_off holds every column as a 16-bit offset from its row
+  DArr<int16_t> sl_off;
   DArr<int> sl_ptr, sl_len, sl_col, sl_map;
   DArr<double> sv64;
   DArr<float> sv32;
@@ -2812,6 +2836,23 @@ void cpr_build_hierarchy(CprSolver* C, const BlockSystem& S, const std::vector<d
       lv->sl_ptr.upload(ptr, s);
       lv->sl_len.upload(len, s);
       lv->sl_col.upload(cl, s);
+      // the same columns as 16-bit offsets from the row, when they all fit
+      // (structured grids: +-1, +-nx, +-nx ny), for the sweeps
+      std::vector<int16_t> d16(off, 0);
+      bool fits = true;
+      for (int q = 0; q < ns && fits; ++q)
+        for (int r = 64 * q; r < std::min(n, 64 * q + 64) && fits; ++r)
+          for (int k = 0; k < len[q]; ++k) {
+            const size_t slot = (size_t)ptr[q] + 64 * (size_t)k + (r - 64 * q);
+            const int dlt = cl[slot] - r;
+            if (dlt < -32768 || dlt > 32767) {
+              fits = false;
+              break;
+            }
+            d16[slot] = (int16_t)dlt;
+          }
+      lv->sell16 = fits;
+      if (fits) lv->sl_off.upload(d16, s);
       lv->sl_map.upload(map, s);
     }
     if (l + 1 < H.size()) {
@@ -3225,7 +3266,10 @@ static void with_lpr(int lpr, F&& f) {
 template <class T>
 static void level_sweep(AmgLevel& a, const T* b, const T* x, T* out, int mode, hipStream_t s) {
   using V = Lv<T>;
-  if (sweeps_from_sell(a))
+  if (sweeps_from_sell(a) && a.sell16)
+    hipLaunchKernelGGL((k_sell_sweep16<T>), dim3(grid_for(a.n)), dim3(256), 0, s, a.n, a.sl_ptr.p, a.sl_len.p,
+                       a.sl_off.p, V::sv(a).p, V::d(a).p, b, x, out, mode);
+  else if (sweeps_from_sell(a))
     hipLaunchKernelGGL((k_sell_sweep<T>), dim3(grid_for(a.n)), dim3(256), 0, s, a.n, a.sl_ptr.p, a.sl_len.p,
                        a.sl_col.p, V::sv(a).p, V::d(a).p, b, x, out, mode);
   else
