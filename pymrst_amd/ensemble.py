@@ -294,7 +294,7 @@ def _run_level(model, st0, st, D, t, dtm, cuts, it, its_step, restart, vol, max_
 
 
 def simulate_ensemble(model: "EnsembleModel", state0, timesteps: Sequence[float], max_cuts: int = 6,
-                      observer=None, keep_device: bool = False, compact: float = 0.3, volumes=None):
+                      observer=None, keep_device: bool = False, compact: float = 0.2, volumes=None):
     """Report steps with per-member time stepping.  Each Newton iteration is
     one union assembly and one (segmented) solve over the members still
     iterating; a member that converges accepts its ministep and starts the
