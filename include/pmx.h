@@ -53,6 +53,8 @@ typedef struct pmx_solver_options {
   int32_t level_shape;      /* -1 (default): shapes as above; 0-6: every AMG level above the coarse cycle
                                takes this shape (0 sweeps from the ELL copy where the level has one,
                                1-3 CSR with 4, 8, 16 lanes per row, 4-6 the same with folded launches) */
+  int32_t dot_shape;        /* 0 (default): the projections' shape as above; 2, 4, 8 (elements per thread,
+                               shuffle sums) or 104, 108 (4 or 8, LDS sums): that shape */
 } pmx_solver_options;
 
 typedef struct pmx_solve_info {

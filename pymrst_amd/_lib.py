@@ -39,6 +39,7 @@ class SolverOptions(ctypes.Structure):
         ("line_segments", ctypes.c_int32),
         ("autotune", ctypes.c_int32),
         ("level_shape", ctypes.c_int32),
+        ("dot_shape", ctypes.c_int32),
     ]
 
 

@@ -89,6 +89,7 @@ static LinearSolverOptions to_opts(const pmx_solver_options* o) {
   r.line_segments = o->line_segments;
   r.autotune = o->autotune;
   r.level_shape = o->level_shape;
+  r.dot_shape = o->dot_shape;
   return r;
 }
 
@@ -134,6 +135,7 @@ void pmx_solver_defaults(pmx_solver_options* o) {
   o->line_segments = d.line_segments;
   o->autotune = d.autotune;
   o->level_shape = d.level_shape;
+  o->dot_shape = d.dot_shape;
 }
 
 int pmx_ow_create(int32_t device, void* hip_stream, const double* fluid, pmx_ow** out) {

@@ -89,6 +89,7 @@ struct LinearSolverOptions {
                              // first setup of each size class, keep the fastest (process-wide table)
   int level_shape = -1;      // >= 0: every level above the coarse cycle takes this candidate shape
                              // (0 ELL, 1-3 CSR with 4/8/16 lanes, 4-6 the same folded), for tests
+  int dot_shape = 0;         // > 0: the projections take this shape (2, 4, 8; 104, 108), for tests
 };
 
 struct SolveInfo {

@@ -3805,7 +3805,13 @@ int cpr_solve(CprSolver* C, const BlockSystem& S, const double* b, double* x, co
   double* hscr = C->hmat.p;
   const int nb = grid_for(n);
   const int rb = red_blocks(n);
-  if (C->dot2_elems == 0 || C->dot2_n != n || (!o.autotune && C->dot2_elems != kDot2Elems)) {
+  if (o.dot_shape > 0) {
+    PMX_CHECK(o.dot_shape == 2 || o.dot_shape == 4 || o.dot_shape == 8 || o.dot_shape == 104 ||
+                  o.dot_shape == 108,
+              "dot_shape is 0 or one of 2, 4, 8, 104, 108");
+    C->dot2_elems = o.dot_shape;
+    C->dot2_n = 0;   // choose again when the forced shape is lifted
+  } else if (C->dot2_elems == 0 || C->dot2_n != n || (!o.autotune && C->dot2_elems != kDot2Elems)) {
     C->dot2_elems = o.autotune ? dot2_shape(C, n, m, ld, s) : kDot2Elems;
     C->dot2_n = n;
   }

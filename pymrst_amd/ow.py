@@ -120,6 +120,7 @@ class CPRSolver:
     # by timing at the first setup of each size class
     autotune: bool = False
     level_shape: int = -1      # 0-6: one launch shape for every AMG level (testing)
+    dot_shape: int = 0         # 2, 4, 8, 104, 108: the projections' shape (testing)
     # a pressure-stage solver with solve_linear_system(A_p, r) -> x, called on
     # the equilibrated A_p = W J P in every preconditioner application
     # instead of the device AMG (linear_solvers.py:124-131, 253-256)
@@ -174,6 +175,7 @@ class CPRSolver:
         o.line_segments = 1 if self.line_segments else 0
         o.autotune = 1 if self.autotune else 0
         o.level_shape = self.level_shape
+        o.dot_shape = self.dot_shape
         return o
 
 

@@ -1,9 +1,10 @@
 """Launch-shape autotuning: every candidate shape of an AMG level (sweeps
 from the ELL copy; CSR with 4, 8 or 16 lanes per row; the same with the
 folded launches) gives the preconditioner application of the default shape
-to rounding, and an autotuned run gives the untuned run's Newton
-iterations and states, with every level tuned and the choices kept in the
-process-wide table for the next system of the same size."""
+to rounding, every shape of the Gram-Schmidt projections gives the default's
+solve, and an autotuned run gives the untuned run's Newton iterations and
+states, with every level tuned and the choices kept in the process-wide
+table for the next system of the same size."""
 import ctypes
 
 import numpy as np
@@ -106,3 +107,37 @@ def test_autotuned_run_matches_the_untuned_run():
     assert _entries(m2.lib) == entries
     spmv2, lv2 = _shapes(m2)
     assert spmv2 == spmv and np.array_equal(lv2[:, 1:4], lv[:, 1:4])
+
+
+def test_every_projection_shape_gives_the_same_solve():
+    """The Gram-Schmidt projections in each shape (2, 4, 8 elements per
+    thread with shuffle sums; 4 or 8 with the sums through LDS) give the
+    default shape's FGMRES solve: the same iteration count within one and
+    the same increment to the solve's tolerance."""
+    from pymrst_amd import _lib
+    from pymrst_amd.ow import CPRSolver, DeviceOWState
+    from pymrst_amd.workloads import build_model, field_waterflood
+    case = field_waterflood(dims=(40, 40, 20), steps=1, step_days=10.0, inj_rate=2e-3)
+    solver = CPRSolver(precision=32, rtol=1e-10, local_sweeps=5)
+    md, s0 = build_model(case, solver)
+    dev = torch.device("cuda")
+    d0 = DeviceOWState.from_host(s0, dev)
+    d = d0.copy()
+    d.p += 3e5 * torch.rand(d.p.shape, dtype=torch.float64, device=dev,
+                            generator=torch.Generator(device=dev).manual_seed(2))
+    dt = float(case.dts[0])
+    md.assemble_device(d0, d, dt, md._cdp_dev)
+    _, pmean = md.converged(dt, d)
+    out = {}
+    for shape in [8, 2, 4, 104, 108]:
+        solver.dot_shape = shape
+        info = _lib.SolveInfo()
+        print(f"projection shape {shape}", flush=True)
+        rc = md.lib.pmx_ow_solve(md.h, ctypes.byref(solver.options()), pmean, _lib.ptr(md._dx), ctypes.byref(info))
+        assert rc == 0
+        assert md.launch_shapes()["projection_elems"] == shape
+        out[shape] = (int(info.iterations), md._dx.cpu().numpy().copy())
+    its0, dx0 = out[8]
+    for shape, (its, dx) in out.items():
+        assert abs(its - its0) <= 1, f"shape {shape}: {its} iterations against {its0}"
+        assert np.abs(dx - dx0).max() <= 1e-7 * np.abs(dx0).max(), f"shape {shape}"
